@@ -1,0 +1,61 @@
+/*
+ * CPU oracle for the GAMERA-OP per-step MHD update -- TEST INFRASTRUCTURE.
+ *
+ * A C restatement of the reference solver's time step (src/solver_solve.c and
+ * src/solver_func.c), used only by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The
+ * product path never links or calls it.
+ *
+ * The oracle keeps the reference's data layout: one contiguous block per field,
+ * [f][i][j][k] with k fastest, NI = ni + 2*4 + 1 points per axis.  Field
+ * indices follow the reference enums (src/config.h:161-499).  The caller fills
+ * the geometry arrays (geo, rec, coordinates, background-field arrays) and the
+ * initial primitive state, then calls go_prologue() once and go_step().
+ */
+#ifndef GAMERA_ORACLE_H
+#define GAMERA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct go_solver go_solver;
+
+/* array selectors for go_array() */
+enum {
+  GO_GAS = 0,   /* NF_gas = 82 fields */
+  GO_GEM = 1,   /* NF_gem = 97 fields */
+  GO_GEO = 2,   /* NF_geo = 28 fields */
+  GO_REC = 3,   /* NF_rec = 42 fields */
+  GO_X1 = 4, GO_X2, GO_X3,
+  GO_X1C, GO_X2C, GO_X3C,
+  GO_X1CTR, GO_X2CTR, GO_X3CTR,
+  GO_DX1, GO_DX2, GO_DX3,
+};
+
+/* boundary kinds per (dimension, side) */
+enum { GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2 };
+
+/* coord: 0 Cartesian, 2 spherical (selects the geometric source terms). */
+go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],
+                     double CA);
+void go_destroy(go_solver *s);
+double *go_array(go_solver *s, int which);
+int64_t go_npoints(const go_solver *s);
+
+/* solve() prologue (solver_solve.c:25-99): ghost fill, BCs, derived
+   variables, X_p := X, dt0 := get_dt(). */
+void go_prologue(go_solver *s);
+/* n AB2 steps of the solve() loop (solver_solve.c:102-529). */
+void go_step(go_solver *s, int n);
+
+/* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max */
+double go_scalar(const go_solver *s, int which);
+void go_set_scalar(go_solver *s, int which, double v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

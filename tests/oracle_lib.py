@@ -1,0 +1,103 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so).
+
+The oracle is the checker: tests load it, hand it the same geometry and
+initial state as the device solver and compare.  Arrays use the reference
+layout: field-major [f][i][j][k], NI = ni + 9 points per axis.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
+GO_GAS, GO_GEM, GO_GEO, GO_REC = 0, 1, 2, 3
+COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr",
+               "x3ctr", "dx1", "dx2", "dx3"]
+BC_PERIODIC, BC_SYMMETRIC, BC_NONE = 0, 1, 2
+
+_libs = {}
+
+
+def load(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(ORACLE_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library {path} is not built (run make -C oracle)")
+    lib = ctypes.CDLL(path)
+    lib.go_create.restype = ctypes.c_void_p
+    lib.go_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_int), ctypes.c_double]
+    lib.go_destroy.argtypes = [ctypes.c_void_p]
+    lib.go_array.restype = ctypes.POINTER(ctypes.c_double)
+    lib.go_array.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.go_npoints.restype = ctypes.c_int64
+    lib.go_npoints.argtypes = [ctypes.c_void_p]
+    lib.go_prologue.argtypes = [ctypes.c_void_p]
+    lib.go_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.go_scalar.restype = ctypes.c_double
+    lib.go_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.go_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    _libs[name] = lib
+    return lib
+
+
+class Oracle:
+    """One oracle solver instance (reference layout, single rank)."""
+
+    def __init__(self, ni, nj, nk, coord=0, bc=(0,) * 6, CA=10.0, fast=False):
+        self.lib = load(fast)
+        arr = (ctypes.c_int * 6)(*bc)
+        self.h = self.lib.go_create(ni, nj, nk, coord, arr, CA)
+        if not self.h:
+            raise MemoryError("go_create failed")
+        self.shape = (ni + 9, nj + 9, nk + 9)
+        self.n3 = int(self.lib.go_npoints(self.h))
+
+    def _view(self, which, nf):
+        p = self.lib.go_array(self.h, which)
+        return np.ctypeslib.as_array(p, shape=(nf,) + self.shape)
+
+    @property
+    def gas(self):
+        return self._view(GO_GAS, NF_GAS)
+
+    @property
+    def gem(self):
+        return self._view(GO_GEM, NF_GEM)
+
+    @property
+    def geo(self):
+        return self._view(GO_GEO, NF_GEO)
+
+    @property
+    def rec(self):
+        return self._view(GO_REC, NF_REC)
+
+    def coord(self, name):
+        p = self.lib.go_array(self.h, 4 + COORD_NAMES.index(name))
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def prologue(self):
+        self.lib.go_prologue(self.h)
+
+    def step(self, n=1):
+        self.lib.go_step(self.h, n)
+
+    def scalar(self, name):
+        return self.lib.go_scalar(self.h, ["dt", "dt0", "time_sim", "divB_max"].index(name))
+
+    def set_scalar(self, name, v):
+        self.lib.go_set_scalar(self.h, ["dt", "dt0", "time_sim", "divB_max"].index(name), v)
+
+    def close(self):
+        if self.h:
+            self.lib.go_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
