@@ -1,0 +1,183 @@
+"""Python bindings for the GAMERA-OP MI355X solver (ctypes over the C ABI).
+
+Two shared libraries are loaded from this directory:
+  libgamera_op.so    the HIP device solver (include/gamera_op.h)
+  libgamera_host.so  the C host facade: reference-compatible globals,
+                     geometry, problem plugins, initialize_solver/solve/
+                     finalize_solver (reference src/solver.h:17-68)
+
+The host facade is global state, one problem at a time, as in the reference.
+Loading fails loudly when the libraries are missing: there is no CPU fallback
+on the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_OP = os.path.join(HERE, "libgamera_op.so")
+LIB_HOST = os.path.join(HERE, "libgamera_host.so")
+
+# seeded problem generators (gamera-op_amd/problems/gk_seeded.c)
+SINUSOID, SINUSOID_BLAST, DIPOLE_SHELL, SINUSOID_SPHERE = 0, 1, 2, 3
+
+NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
+COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr",
+               "dx1", "dx2", "dx3"]
+
+_host = None
+_op = None
+
+
+def _load():
+    global _host, _op
+    if _host is not None:
+        return _host, _op
+    for p in (LIB_OP, LIB_HOST):
+        if not os.path.exists(p):
+            raise RuntimeError(f"{p} is not built; run __graft_entry__.build()")
+    _op = ctypes.CDLL(LIB_OP, mode=ctypes.RTLD_GLOBAL)
+    h = ctypes.CDLL(LIB_HOST, mode=ctypes.RTLD_GLOBAL)
+    h.gkh_setup.restype = ctypes.c_int
+    h.gkh_setup.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    h.gkh_ptr.restype = ctypes.POINTER(ctypes.c_double)
+    h.gkh_ptr.argtypes = [ctypes.c_int, ctypes.c_int]
+    h.gkh_sizes.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    h.gkh_bc_table.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    h.gkh_CA.restype = ctypes.c_double
+    h.gkh_b0_mode.restype = ctypes.c_int
+    h.gkh_device_open.restype = ctypes.c_int
+    h.gkh_device_open.argtypes = [ctypes.c_int]
+    h.gkh_device_fetch.restype = ctypes.c_int
+    h.gkh_device.restype = ctypes.c_void_p
+    h.solve.restype = ctypes.c_int
+    h.solve.argtypes = [ctypes.c_int]
+    h.finalize_solver.restype = ctypes.c_int
+    _op.gk_step.restype = ctypes.c_int
+    _op.gk_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    _op.gk_get_scalars.restype = ctypes.c_int
+    _op.gk_get_scalars.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    _op.gk_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    _op.gk_kernel_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    _op.gk_debug_fetch.restype = ctypes.c_int
+    _op.gk_debug_fetch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)]
+    _op.gk_device_bytes.restype = ctypes.c_size_t
+    _op.gk_device_bytes.argtypes = [ctypes.c_void_p]
+    _host = h
+    return _host, _op
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points declared in include/gamera_op.h."""
+    return ["gk_create", "gk_destroy", "gk_set_state", "gk_prologue", "gk_step",
+            "gk_get_state", "gk_get_scalars", "gk_set_time", "gk_enable_timing",
+            "gk_kernel_times", "gk_device_bytes"]
+
+
+class GkError(RuntimeError):
+    pass
+
+
+class Problem:
+    """Host facade state for one seeded problem (global, one at a time)."""
+
+    def __init__(self, kind, seed, ni, nj, nk, coord=None, full_b0=False):
+        self.h, self.op = _load()
+        if coord is None:
+            coord = 2 if kind == DIPOLE_SHELL else 0
+        rc = self.h.gkh_setup(kind, seed, ni, nj, nk, coord, 1 if full_b0 else 0)
+        if rc != 0:
+            raise GkError(f"gkh_setup failed ({rc})")
+        sz = (ctypes.c_int * 6)()
+        self.h.gkh_sizes(sz)
+        self.n = (sz[0], sz[1], sz[2])
+        self.shape = (sz[3], sz[4], sz[5])
+        self.kind, self.seed, self.coord = kind, seed, coord
+        self.CA = self.h.gkh_CA()
+
+    def _view(self, which, f):
+        p = self.h.gkh_ptr(which, f)
+        if not p:
+            raise GkError(f"array {which}/{f} is not stored on the host")
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def gas(self, f):
+        return self._view(0, f)
+
+    def gem(self, f):
+        return self._view(1, f)
+
+    def geo(self, f):
+        return self._view(2, f)
+
+    def rec(self, f):
+        return self._view(3, f)
+
+    def coord_array(self, name):
+        return self._view(4 + COORD_NAMES.index(name), 0)
+
+    def bc_table(self):
+        t = (ctypes.c_int * 12)()
+        self.h.gkh_bc_table(t)
+        return np.array(t[:], dtype=int).reshape(2, 6)
+
+    def b0_mode(self):
+        return self.h.gkh_b0_mode()
+
+    # --- device path -----------------------------------------------------
+    def device_open(self, precision=0):
+        rc = self.h.gkh_device_open(precision)
+        if rc != 0:
+            raise GkError(f"gkh_device_open failed ({rc})")
+        self.dev = self.h.gkh_device()
+
+    def device_step(self, n):
+        rc = self.op.gk_step(self.dev, n)
+        if rc != 0:
+            raise GkError(f"gk_step failed ({rc})")
+
+    def device_fetch(self):
+        rc = self.h.gkh_device_fetch()
+        if rc != 0:
+            raise GkError(f"gkh_device_fetch failed ({rc})")
+
+    def device_scalars(self):
+        out = (ctypes.c_double * 6)()
+        rc = self.op.gk_get_scalars(self.dev, out)
+        if rc != 0:
+            raise GkError(f"gk_get_scalars failed ({rc})")
+        return dict(dt=out[0], dt0=out[1], time=out[2], divB=out[3], next_dt=out[4],
+                    nonfinite=out[5])
+
+    def enable_timing(self, on=True):
+        self.op.gk_enable_timing(self.dev, 1 if on else 0)
+
+    def kernel_times(self):
+        n = ctypes.c_int(32)
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_double * 32)()
+        self.op.gk_kernel_times(self.dev, names, ms, ctypes.byref(n))
+        return {names[q].decode(): ms[q] for q in range(min(n.value, 32))}
+
+    def debug_fetch(self, which, field):
+        out = np.zeros(self.shape)
+        rc = self.op.gk_debug_fetch(self.dev, which, field,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc != 0:
+            raise GkError(f"gk_debug_fetch failed ({rc})")
+        return out
+
+    def device_bytes(self):
+        return int(self.op.gk_device_bytes(self.dev))
+
+    def solve(self, nt):
+        rc = self.h.solve(nt)
+        if rc != 0:
+            raise GkError(f"solve failed ({rc})")
+
+    def close(self):
+        self.h.finalize_solver()

@@ -1,0 +1,361 @@
+/*
+ * Solver entry points of the host facade: initialize_solver, solve,
+ * finalize_solver (reference src/solver_init.c:352-450 and
+ * src/solver_solve.c:23-536), backed by the gamera_op device solver.
+ *
+ * solve(nt) keeps the reference's semantics on a single GPU: the prologue
+ * (ghost fill, boundary conditions, derived variables, X_p := X, dt0) and then
+ * nt AB2 steps, stopping early once time_sim exceeds time_stop.  The state
+ * lives in device memory between calls; the host arrays are refreshed after
+ * solve() returns and whenever a dump is due.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "common.h"
+#include "config.h"
+#include "curvilinear.h"
+#include "gk_host.h"
+#include "log.h"
+#include "problem.h"
+#include "solver.h"
+
+/* optional problem hook: rewrite x1/x2/x3 before the metrics are computed */
+int gk_problem_custom_grid(void) __attribute__((weak));
+
+static gk_solver *g_dev = NULL;
+static gk_tables g_tab;
+static int g_precision = 0;
+static int g_dev_fresh = 1; /* prologue pending */
+
+int gk_host_dump_hook(const char *base) __attribute__((weak));
+
+static unsigned char gas_keep[NF_gas], gem_keep[NF_gem];
+
+int allocate_solver() {
+  memset(gas_keep, 0, sizeof gas_keep);
+  memset(gem_keep, 0, sizeof gem_keep);
+  for (int f = 0; f < NF_gas_store; f++) gas_keep[f] = 1;
+  for (int f = 0; f < NF_gem_store; f++) gem_keep[f] = 1;
+  for (int f = LAi; f <= dmagsf_G_x3dir; f++) gem_keep[f] = 1;
+  for (int f = magtot_b1; f <= magtot_b3; f++) gem_keep[f] = 1;
+  double ****arrs[12];
+  (void)arrs;
+  x1 = gk_alloc3(); x2 = gk_alloc3(); x3 = gk_alloc3();
+  x1c = gk_alloc3(); x2c = gk_alloc3(); x3c = gk_alloc3();
+  x1ctr = gk_alloc3(); x2ctr = gk_alloc3(); x3ctr = gk_alloc3();
+  dx1 = gk_alloc3(); dx2 = gk_alloc3(); dx3 = gk_alloc3();
+  geo = gk_alloc4(NF_geo, NULL);
+  rec = gk_alloc4(NF_rec, NULL);
+  gem = gk_alloc4(NF_gem, gem_keep);
+  gas = (double *****)calloc(NS1, sizeof(double ****));
+  if (gas) gas[0] = gk_alloc4(NF_gas, gas_keep);
+  if (!x1 || !x2 || !x3 || !x1c || !x2c || !x3c || !x1ctr || !x2ctr || !x3ctr || !dx1 || !dx2 ||
+      !dx3 || !geo || !rec || !gem || !gas || !gas[0])
+    return -1;
+  for (int f = 0; f < NF_geo; f++)
+    if (!geo[f]) return -1;
+  for (int f = 0; f < NF_rec; f++)
+    if (!rec[f]) return -1;
+  for (int f = 0; f < NF_gem; f++)
+    if (gem_keep[f] && !gem[f]) return -1;
+  for (int f = 0; f < NF_gas; f++)
+    if (gas_keep[f] && !gas[0][f]) return -1;
+  return 0;
+}
+
+int set_domain_decomposition() {
+  gk_default_decomposition();
+  return 0;
+}
+
+int initialize_solver(const char *base_name) {
+  (void)base_name;
+  log_info("Allocating solver");
+  if (allocate_solver() != 0) {
+    log_error("host allocation failed");
+    return -1;
+  }
+  /* restart files are handled by the HDF5 layer when it is present */
+  read_restart = false;
+  time_sim = 0.0;
+  time_sim_start = 0.0;
+  if (set_domain_decomposition() != 0) return -1;
+  if (gk_problem_custom_grid) gk_problem_custom_grid();
+  if (set_geometry_arrays() != 0) return -1;
+  log_info("Solver initialized");
+  return 0;
+}
+
+int finalize_solver() {
+  if (g_dev) {
+    gk_destroy(g_dev);
+    g_dev = NULL;
+  }
+  gk_tables_free(&g_tab);
+  double ***a3[12] = {x1, x2, x3, x1c, x2c, x3c, x1ctr, x2ctr, x3ctr, dx1, dx2, dx3};
+  for (int n = 0; n < 12; n++) gk_free3(a3[n]);
+  x1 = x2 = x3 = x1c = x2c = x3c = x1ctr = x2ctr = x3ctr = dx1 = dx2 = dx3 = NULL;
+  gk_free4(geo, NF_geo);
+  gk_free4(rec, NF_rec);
+  gk_free4(gem, NF_gem);
+  if (gas) {
+    gk_free4(gas[0], NF_gas);
+    free(gas);
+  }
+  geo = rec = gem = NULL;
+  gas = NULL;
+  g_dev_fresh = 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* host versions of the derived-variable and CFL routines
+   (src/solver_func.c:255-383), used on downloaded state */
+void get_derived_variables() {
+  for (int i = isg; i <= ieg; i++)
+    for (int j = jsg; j <= jeg; j++)
+      for (int k = ksg; k <= keg; k++) {
+        gem[mags_b1][i][j][k] = gem[mag_bi][i][j][k] * (x1[i + 1][j][k] - x1ctr[i][j][k]) / dx1[i][j][k] +
+                                gem[mag_bi][i + 1][j][k] * (x1ctr[i][j][k] - x1[i][j][k]) / dx1[i][j][k];
+        gem[mags_b2][i][j][k] = gem[mag_bj][i][j][k] * (x2[i][j + 1][k] - x2ctr[i][j][k]) / dx2[i][j][k] +
+                                gem[mag_bj][i][j + 1][k] * (x2ctr[i][j][k] - x2[i][j][k]) / dx2[i][j][k];
+        gem[mags_b3][i][j][k] = gem[mag_bk][i][j][k] * (x3[i][j][k + 1] - x3ctr[i][j][k]) / dx3[i][j][k] +
+                                gem[mag_bk][i][j][k + 1] * (x3ctr[i][j][k] - x3[i][j][k]) / dx3[i][j][k];
+        for (int m = 0; m < 3; m++)
+          gem[magtot_b1 + m][i][j][k] = gem[mags_b1 + m][i][j][k] + gem[B0_b1 + m][i][j][k];
+      }
+  double dmax = 0.0;
+  for (int i = is; i <= ie; i++)
+    for (int j = js; j <= je; j++)
+      for (int k = ks; k <= ke; k++) {
+        double d = ((gem[mag_bi][i + 1][j][k] + gem[B0AfaceI_b1][i + 1][j][k]) * geo[face_idir][i + 1][j][k] -
+                    (gem[mag_bi][i][j][k] + gem[B0AfaceI_b1][i][j][k]) * geo[face_idir][i][j][k]) / geo[vol_center][i][j][k] +
+                   ((gem[mag_bj][i][j + 1][k] + gem[B0AfaceJ_b2][i][j + 1][k]) * geo[face_jdir][i][j + 1][k] -
+                    (gem[mag_bj][i][j][k] + gem[B0AfaceJ_b2][i][j][k]) * geo[face_jdir][i][j][k]) / geo[vol_center][i][j][k] +
+                   ((gem[mag_bk][i][j][k + 1] + gem[B0AfaceK_b3][i][j][k + 1]) * geo[face_kdir][i][j][k + 1] -
+                    (gem[mag_bk][i][j][k] + gem[B0AfaceK_b3][i][j][k]) * geo[face_kdir][i][j][k]) / geo[vol_center][i][j][k];
+        if (fabs(d) > dmax) dmax = fabs(d);
+      }
+  divB_max = dmax;
+}
+
+double get_dt() {
+  double m = INFINITY;
+  for (int i = is; i <= ie; i++)
+    for (int j = js; j <= je; j++)
+      for (int k = ks; k <= ke; k++) {
+        double b1 = gem[magtot_b1][i][j][k], b2 = gem[magtot_b2][i][j][k], b3 = gem[magtot_b3][i][j][k];
+        double rho = gas[0][gas_rho][i][j][k];
+        double Va = sqrt((b1 * b1 + b2 * b2 + b3 * b3) / rho);
+        double Vae = Va * CA / (sqrt(Va * Va + CA * CA));
+        double v1 = gas[0][gas_v1][i][j][k], v2 = gas[0][gas_v2][i][j][k], v3 = gas[0][gas_v3][i][j][k];
+        double Vf = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+        double cs = sqrt(gamma_val * gas[0][gas_p][i][j][k] / rho);
+        double V = Vf + sqrt(cs * cs + Vae * Vae);
+        double d = CFL / (V / geo[dxi][i][j][k] + V / geo[dxj][i][j][k] + V / geo[dxk][i][j][k]);
+        m = fmin(m, d);
+      }
+  return m;
+}
+
+void check_positivity() {
+  for (int i = is; i <= ie; i++)
+    for (int j = js; j <= je; j++)
+      for (int k = ks; k <= ke; k++) {
+        if (gas[0][gas_rho][i][j][k] < rho_floor) {
+          gas[0][gas_rho][i][j][k] = rho_floor;
+          gas[0][gas_p][i][j][k] = p_floor;
+        }
+        if (gas[0][gas_p][i][j][k] < p_floor) gas[0][gas_p][i][j][k] = p_floor;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* device plumbing */
+static int record_bcs(void) {
+  memset(gk_bc_kind, 0, sizeof gk_bc_kind);
+  gk_bc_record_only = 1;
+  boundary_conditions();
+  gk_bc_record_only = 0;
+  return 0;
+}
+
+int gkh_device_open(int precision) {
+  if (g_dev) {
+    gk_destroy(g_dev);
+    g_dev = NULL;
+  }
+  gk_tables_free(&g_tab);
+  record_bcs();
+  if (gk_tables_build(&g_tab) != 0) return GK_ERR_ARG;
+  gk_grid_desc d;
+  int rc = gk_fill_desc(&d, &g_tab, precision);
+  if (rc != GK_OK) {
+    log_error("device solver: unsupported boundary kind or background field");
+    return rc;
+  }
+  rc = gk_create(&d, &g_dev);
+  if (rc != GK_OK) return rc;
+  g_precision = precision;
+  const double *prim[6], *bf[3];
+  for (int m = 0; m < 6; m++) prim[m] = &gas[0][m][0][0][0];
+  for (int m = 0; m < 3; m++) bf[m] = &gem[m][0][0][0];
+  rc = gk_set_state(g_dev, prim, bf, NULL, NULL);
+  if (rc != GK_OK) return rc;
+  rc = gk_prologue(g_dev, 0.0);
+  if (rc != GK_OK) return rc;
+  rc = gk_set_time(g_dev, time_sim);
+  g_dev_fresh = 0;
+  return rc;
+}
+
+gk_solver *gkh_device(void) { return g_dev; }
+
+int gkh_device_fetch(void) {
+  if (!g_dev) return GK_ERR_STATE;
+  double *prim[6], *bf[3], *pp[6], *bp[3];
+  for (int m = 0; m < 6; m++) {
+    prim[m] = &gas[0][m][0][0][0];
+    pp[m] = &gas[0][gas_rho_p + m][0][0][0];
+  }
+  for (int m = 0; m < 3; m++) {
+    bf[m] = &gem[m][0][0][0];
+    bp[m] = &gem[mag_bi_p + m][0][0][0];
+  }
+  int rc = gk_get_state(g_dev, prim, bf, pp, bp);
+  if (rc != GK_OK) return rc;
+  double sc[6];
+  rc = gk_get_scalars(g_dev, sc);
+  if (rc != GK_OK) return rc;
+  dt = sc[0];
+  dt0 = sc[1];
+  time_sim = sc[2];
+  divB_max = sc[3];
+  get_derived_variables();
+  return GK_OK;
+}
+
+int solve(int nt) {
+  int rc;
+  if (g_dev_fresh || !g_dev) {
+    rc = gkh_device_open(g_precision);
+    if (rc != GK_OK) {
+      log_error("Error: device solver setup failed (%d).", rc);
+      return -1;
+    }
+  }
+  time_sim_start = time_sim;
+  const int per_step = time_stop < 1e300 && time_stop > 0.0;
+  if (!per_step) {
+    rc = gk_step(g_dev, nt);
+    if (rc != GK_OK) return -1;
+  } else {
+    for (int n = 1; n <= nt; n++) {
+      rc = gk_step(g_dev, 1);
+      if (rc != GK_OK) return -1;
+      double sc[6];
+      gk_get_scalars(g_dev, sc);
+      if (sc[5] != 0.0) log_error("non-finite state detected at step %d", n);
+      if (time_sim_start + 0.0 < sc[2] && sc[2] >= 0.1 * output_interval * log_seq_num) {
+        log_info("Status at step %d: rank 0, Time %.15f, dt %.15f, max_divB for this rank is %.4e", n,
+                 sc[2], sc[0], sc[3]);
+        log_seq_num++;
+      }
+      if (sc[2] > time_stop) {
+        log_info("Step %d, Time %f, dt %f", n, sc[2], sc[0]);
+        break;
+      }
+    }
+  }
+  rc = gkh_device_fetch();
+  return rc == GK_OK ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* programmatic set-up used by the Python bindings, tests and the benchmark:
+   the reference main() sequence (src/main.c:30-63) without a YAML file */
+extern int gk_problem_kind;
+extern uint64_t gk_problem_seed;
+extern int gk_problem_seed_set;
+
+static gk_tables g_probe;
+void gkh_table_reset(void);
+int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int full_b0) {
+  if (gas) finalize_solver();
+  gkh_table_reset();
+  gk_problem_kind = kind;
+  gk_problem_seed = seed;
+  gk_problem_seed_set = 1;
+  gk_coord_type = coord;
+  B10 = B20 = B30 = Field_Default;
+  Source1_B0 = Source2_B0 = Source3_B0 = Field_Default;
+  A1 = A2 = A3 = Field_Default;
+  time_sim = 0.0;
+  set_problem_config();
+  const int dims[3] = {1, 1, 1};
+  if (gk_config_set_sizes(ni, nj, nk, dims) != 0) return -1;
+  config.proc_coords[0] = config.proc_coords[1] = config.proc_coords[2] = 0;
+  gk_config_finish();
+  Nt = 0;
+  time_stop = 1e300;
+  output_interval = 1e300;
+  if (initialize_solver(BASE_DAT_NAME) != 0) return -2;
+  getweights();
+  problem_init();
+  gk_b0_full_arrays = full_b0;
+  set_background_field();
+  if (gk_b0.mode < 0) return -3;
+  record_bcs();
+  g_dev_fresh = 1;
+  return 0;
+}
+
+/* host array access for the bindings: 0 gas f, 1 gem f, 2 geo f, 3 rec f,
+   4.. coordinates x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
+double *gkh_ptr(int which, int f) {
+  double ***c3[12] = {x1, x2, x3, x1c, x2c, x3c, x1ctr, x2ctr, x3ctr, dx1, dx2, dx3};
+  switch (which) {
+    case 0: return (f >= 0 && f < NF_gas && gas && gas[0][f]) ? &gas[0][f][0][0][0] : NULL;
+    case 1: return (f >= 0 && f < NF_gem && gem && gem[f]) ? &gem[f][0][0][0] : NULL;
+    case 2: return (f >= 0 && f < NF_geo && geo) ? &geo[f][0][0][0] : NULL;
+    case 3: return (f >= 0 && f < NF_rec && rec) ? &rec[f][0][0][0] : NULL;
+    default:
+      if (which >= 4 && which < 16 && c3[which - 4]) return &c3[which - 4][0][0][0];
+      return NULL;
+  }
+}
+
+void gkh_sizes(int out[6]) {
+  out[0] = config.ni;
+  out[1] = config.nj;
+  out[2] = config.nk;
+  out[3] = config.NI;
+  out[4] = config.NJ;
+  out[5] = config.NK;
+}
+
+void gkh_bc_table(int out[2][6]) { memcpy(out, gk_bc_kind, sizeof gk_bc_kind); }
+
+double gkh_CA(void) { return CA; }
+int gkh_b0_mode(void) { return gk_b0.mode; }
+
+/* per-axis device tables, for checking them against the 3-D host geometry:
+   kind 0 geometry factor (q, axis), 1 cell-centre coefficient (axis, q),
+   2 reconstruction table (axis) */
+const double *gkh_table(int kind, int a, int b) {
+  if (!g_probe.recw[0]) gk_tables_build(&g_probe);
+  switch (kind) {
+    case 0: return g_probe.geom[a][b];
+    case 1: return g_probe.cc[a][b];
+    default: return g_probe.recw[a];
+  }
+}
+void gkh_table_reset(void) { gk_tables_free(&g_probe); }
+const int *gkh_table_cls(int a, int q, int *ncls) {
+  if (!g_probe.recw[0]) gk_tables_build(&g_probe);
+  *ncls = g_probe.ncls[a][q];
+  return g_probe.cls[a][q];
+}

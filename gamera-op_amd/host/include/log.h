@@ -1,0 +1,29 @@
+/*
+ * log.h -- leveled logging for the host facade.  Problem plugins call the
+ * log_* macros of the reference's logc/log.h:33-38; same names, same printf
+ * semantics, written to the per-rank log file opened by initialize_logging().
+ */
+#ifndef LOG_H
+#define LOG_H
+
+#include <stdbool.h>
+#include <stdio.h>
+
+enum { LOG_TRACE, LOG_DEBUG, LOG_INFO, LOG_WARN, LOG_ERROR, LOG_FATAL };
+
+#define log_trace(...) log_log(LOG_TRACE, __FILE__, __LINE__, __VA_ARGS__)
+#define log_debug(...) log_log(LOG_DEBUG, __FILE__, __LINE__, __VA_ARGS__)
+#define log_info(...) log_log(LOG_INFO, __FILE__, __LINE__, __VA_ARGS__)
+#define log_warn(...) log_log(LOG_WARN, __FILE__, __LINE__, __VA_ARGS__)
+#define log_error(...) log_log(LOG_ERROR, __FILE__, __LINE__, __VA_ARGS__)
+#define log_fatal(...) log_log(LOG_FATAL, __FILE__, __LINE__, __VA_ARGS__)
+
+void log_set_level(int level);
+void log_set_quiet(bool enable);
+int log_add_fp(FILE *fp, int level);
+void log_log(int level, const char *file, int line, const char *fmt, ...)
+    __attribute__((format(printf, 4, 5)));
+
+#define printError(...) fprintf(stderr, __VA_ARGS__)
+
+#endif

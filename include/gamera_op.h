@@ -1,0 +1,130 @@
+/*
+ * gamera_op.h -- C ABI of the MI355X (gfx950) device solver for the
+ * GAMERA-OP per-step ideal-MHD update.
+ *
+ * This is the thin layer the C host facade (initialize_solver / solve /
+ * finalize_solver, src/solver.h:17,26,65 in the reference) calls to run the
+ * time step on the GPU.  It replaces, for a single-GPU domain:
+ *   - the solve() loop body           src/solver_solve.c:102-508
+ *   - the numerical kernels            src/solver_func.c:215-1529
+ *   - the halo exchange                src/boundary_exchange.c:7-201
+ *   - the boundary-condition library   src/boundary_conditions.c:9-593
+ * All entry points take plain pointers and sizes and return 0 on success or a
+ * negative gk_status code; nothing throws across the ABI.  Host arrays use
+ * the reference layout: one contiguous block per field, [i][j][k] with k
+ * fastest, NI = ni + 9 points per axis (4 ghost layers each side plus the
+ * upper-face slot), i.e. the layout of src/array.c:87-114.
+ */
+#ifndef GAMERA_OP_H
+#define GAMERA_OP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GK_OK = 0,
+  GK_ERR_ARG = -1,      /* invalid argument or size */
+  GK_ERR_DEVICE = -2,   /* HIP runtime error */
+  GK_ERR_NOMEM = -3,    /* device allocation failed */
+  GK_ERR_STATE = -4,    /* call out of order (e.g. step before prologue) */
+  GK_ERR_NAN = -5,      /* non-finite state detected by the fused reduction */
+} gk_status;
+
+/* boundary kinds per array, dimension and side */
+enum { GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2 };
+
+/* separable geometry quantities: q(i,j,k) = f[q][0][i] * f[q][1][j] * f[q][2][k] */
+enum {
+  GK_G_EDGE_I = 0, GK_G_EDGE_J, GK_G_EDGE_K,
+  GK_G_FACE_I, GK_G_FACE_J, GK_G_FACE_K,
+  GK_G_VOL, GK_G_DXI, GK_G_DXJ, GK_G_DXK,
+  GK_G_COUNT
+};
+
+/* per-face reconstruction table entry (per axis, indexed by face index) */
+#define GK_RECW_STRIDE 18 /* wL[7] wR[7] c1L c2L c1R c2R */
+
+typedef struct {
+  int ni, nj, nk;          /* active cells per axis */
+  int coord;               /* 0 Cartesian, 2 spherical (geometric sources) */
+  int bc[2][6];            /* [gas|gem][i_lo,i_hi,j_lo,j_hi,k_lo,k_hi] */
+  int precision;           /* 0 FP64 (reference arithmetic), 1 FP32-bulk */
+  double CA;               /* Boris speed of light (src/config.c:85) */
+  /* per-axis tables; axis a has N_a = n_a + 9 entries, indexed like the
+     reference arrays */
+  const double *geom[GK_G_COUNT][3];
+  /* cell-centre B from the two faces (src/solver_func.c:310-322):
+     b[n] = (B[n] * cc[0][n]) / cc[2][n] + (B[n+1] * cc[1][n]) / cc[2][n],
+     cc = {x[n+1] - xctr[n], xctr[n] - x[n], dx[n]} */
+  const double *cell_coef[3][3];
+  /* reconstruction tables of sweep axis a, indexed
+     [((face * ncls[a][0] + class_b) * ncls[a][1] + class_c) * GK_RECW_STRIDE],
+     b < c the transverse axes; cls[a][0|1][t] gives the class of transverse
+     coordinate t (on Cartesian grids the weights depend on the transverse
+     cell widths through roundoff only; one class per distinct width keeps
+     them exact) */
+  const double *recw[3];
+  const int *cls[3][2];
+  int ncls[3][2];
+  /* background field B0 (src/solver_func.c:1828-2077): mode 0 none,
+     1 spatially uniform (constants below) */
+  int b0_mode;
+  double b0_cell[3];
+  double b0_face[3][8];    /* per face direction: b1 b2 b3 bn bsq bn1 bn2 bn3 */
+  double b0_edge[3][3];    /* per edge direction: b1 b2 b3 */
+  double b0_lorentz[3];    /* dmagsf_G, constant */
+} gk_grid_desc;
+
+typedef struct gk_solver gk_solver;
+
+/* Create a solver on the current HIP device. */
+int gk_create(const gk_grid_desc *desc, gk_solver **out);
+int gk_destroy(gk_solver *s);
+
+/* Upload the primitive state (6 cell fields rho v1 v2 v3 p p_S) and the
+   face-normal magnetic field (3 face fields), reference layout, NI*NJ*NK
+   doubles each.  prev_* may be NULL (then X_p := X, as the solve() prologue
+   does); otherwise they carry the AB2 history of a restart. */
+int gk_set_state(gk_solver *s, const double *const prim[6], const double *const bface[3],
+                 const double *const prim_prev[6], const double *const bface_prev[3]);
+
+/* solve() prologue (src/solver_solve.c:25-99): ghost fill + boundary
+   conditions, then dt0 := CFL time step.  With history uploaded, dt0 is the
+   value passed in (restart). */
+int gk_prologue(gk_solver *s, double dt0_restart);
+
+/* Advance n AB2 steps; dt, dt0 and time stay on the device. */
+int gk_step(gk_solver *s, int n);
+
+/* Download state and history (any pointer may be NULL). */
+int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], double *prim_prev[6],
+                 double *bface_prev[3]);
+
+/* scalars: [0] dt of the last step, [1] dt0, [2] time_sim, [3] max|div B|,
+   [4] next CFL time step, [5] non-finite flag */
+int gk_get_scalars(gk_solver *s, double out[6]);
+int gk_set_time(gk_solver *s, double time_sim);
+
+/* Average kernel durations (ms) over the last gk_step call when timing is
+   enabled: names/ms arrays of length *n (in/out). */
+int gk_enable_timing(gk_solver *s, int on);
+int gk_kernel_times(gk_solver *s, const char **names, double *ms, int *n);
+
+/* Copy one intermediate field of the last step to the host (reference
+   layout): which 0..2 face fluxes of sweep i/j/k (field 0..8: rho, rho v1..3,
+   energy, entropy, magnetic stress 1..3), 3..5 face-averaged velocities of
+   sweep i/j/k (field 0..1), 6 edge electric fields (field 0..2), 7 the
+   cell-centred magnetic field of the current state (field 0..2). */
+int gk_debug_fetch(gk_solver *s, int which, int field, double *host);
+
+/* device bytes held by the solver */
+size_t gk_device_bytes(const gk_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -80,6 +80,7 @@ struct go_solver {
   double *gas, *gem, *geo, *rec;
   double *crd[12]; /* x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
   double dt, dt0, time_sim, divB_max;
+  double *dbgF; /* optional per-direction copies of the 9 face fluxes */
 };
 
 #define IDX(i, j, k) ((((int64_t)(i)) * s->NJ + (j)) * s->NK + (k))
@@ -130,7 +131,7 @@ go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],
 
 void go_destroy(go_solver *s) {
   if (!s) return;
-  free(s->gas); free(s->gem); free(s->geo); free(s->rec);
+  free(s->gas); free(s->gem); free(s->geo); free(s->rec); free(s->dbgF);
   for (int n = 0; n < 12; n++) free(s->crd[n]);
   free(s);
 }
@@ -712,6 +713,17 @@ static void step_once(go_solver *s) {
     for (int m = 0; m < 3; m++)
       reconstruct(s, s->gem, dir, mag_bi_p - mag_bi_h, mags_b1_h + m, magsl_b1 + m, magsr_b1 + m);
     face_fluxes(s, dir);
+    if (s->dbgF) {
+      for (int m = 0; m < 6; m++)
+        memcpy(&s->dbgF[(int64_t)(dir * 9 + m) * s->n3], &s->gas[(int64_t)(gascf_rho + m) * s->n3],
+               sizeof(double) * (size_t)s->n3);
+      for (int m = 0; m < 3; m++)
+        memcpy(&s->dbgF[(int64_t)(dir * 9 + 6 + m) * s->n3], &s->gem[(int64_t)(magsf_x1dir + m) * s->n3],
+               sizeof(double) * (size_t)s->n3);
+      for (int m = 0; m < 6; m++)
+        memcpy(&s->dbgF[(int64_t)(27 + dir * 6 + m) * s->n3], &s->gem[(int64_t)(magsl_b1 + m) * s->n3],
+               sizeof(double) * (size_t)s->n3);
+    }
     source_terms(s, dir);
     flux_divergence(s, dir);
   }
@@ -835,6 +847,11 @@ void go_prologue(go_solver *s) {
       for (int k = s->ksg; k <= s->keg + 1; k++)
         for (int m = 0; m < 3; m++) GEM(mag_bi_p + m, i, j, k) = GEM(mag_bi + m, i, j, k);
   s->dt0 = get_dt(s);
+}
+
+double *go_debug_fluxes(go_solver *s) {
+  if (!s->dbgF) s->dbgF = (double *)calloc((size_t)(45 * s->n3), sizeof(double));
+  return s->dbgF;
 }
 
 void go_step(go_solver *s, int n) {

@@ -51,6 +51,12 @@ void go_prologue(go_solver *s);
 /* n AB2 steps of the solve() loop (solver_solve.c:102-529). */
 void go_step(go_solver *s, int n);
 
+/* enable and return per-direction copies of the face fluxes of the last
+   step: 27 fields [dir*9 + m], m = rho, rho v1..3, energy, entropy, magnetic
+   stress 1..3; then 18 fields [27 + dir*6 + m] of the reconstructed cell B,
+   left b1..b3 then right b1..b3 */
+double *go_debug_fluxes(go_solver *s);
+
 /* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max */
 double go_scalar(const go_solver *s, int which);
 void go_set_scalar(go_solver *s, int which, double v);

@@ -41,6 +41,8 @@ def load(fast=False):
     lib.go_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.go_scalar.restype = ctypes.c_double
     lib.go_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.go_debug_fluxes.restype = ctypes.POINTER(ctypes.c_double)
+    lib.go_debug_fluxes.argtypes = [ctypes.c_void_p]
     lib.go_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     _libs[name] = lib
     return lib
@@ -81,6 +83,10 @@ class Oracle:
     def coord(self, name):
         p = self.lib.go_array(self.h, 4 + COORD_NAMES.index(name))
         return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def debug_fluxes(self):
+        p = self.lib.go_debug_fluxes(self.h)
+        return np.ctypeslib.as_array(p, shape=(45,) + self.shape)
 
     def prologue(self):
         self.lib.go_prologue(self.h)
