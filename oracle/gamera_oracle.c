@@ -394,6 +394,12 @@ static inline double PDMU7(const double fh0[7], const double fp0[7],
   return f - s1 * fmax(0, fabs(df_left) - q0 * df0c);
 }
 
+/* exported for unit tests: one PDMU7 evaluation */
+double go_pdmu7(const double fh0[7], const double fp0[7], const double w[7], double faceim1,
+                double facei, double vol, double dx) {
+  return PDMU7(fh0, fp0, w, faceim1, facei, vol, dx);
+}
+
 /* reconstruct_3dv_{i,j,k} (src/solver_func.c:503-649) for one field array
    base (gas or gem), source field m, limiter field m+pdm, outputs ml, mr. */
 static void reconstruct(go_solver *s, double *base, int dir, int pdm, int m,
@@ -612,6 +618,39 @@ static void face_fluxes(go_solver *s, int dir) {
         for (int m = 0; m < 3; m++)
           GEM(magsf_x1dir + m, i, j, k) = ms[m] - B_sq * (vr[m] - vl[m]);
       }
+}
+
+/* exported for unit tests: the face fluxes of one face (dir 0/1/2) from
+   left/right primitive states, left/right cell B, the B0 face block
+   (b1 b2 b3 bn bsq bn1 bn2 bn3) and the face-normal B; out[9] = rho, rho v1..3,
+   energy, entropy, magnetic stress 1..3.  Runs face_fluxes on a scratch
+   solver of the minimum size. */
+void go_face_flux(int dir, const double L[6], const double R[6], const double bl[3],
+                  const double br[3], const double b0[8], double Bn, double CA, double out[9]) {
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  go_solver *s = go_create(5, 5, 5, 0, bc, CA);
+  if (!s) return;
+  int i = s->is, j = s->js, k = s->ks;
+  for (int m = 0; m < 6; m++) {
+    GAS(gasl_rho + m, i, j, k) = L[m];
+    GAS(gasr_rho + m, i, j, k) = R[m];
+  }
+  for (int m = 0; m < 3; m++) {
+    GEM(magsl_b1 + m, i, j, k) = bl[m];
+    GEM(magsr_b1 + m, i, j, k) = br[m];
+  }
+  for (int q = 0; q < 8; q++) GEM(B0AfaceI_b1 + 8 * dir + q, i, j, k) = b0[q];
+  GEM(mag_bi + dir, i, j, k) = Bn;
+  /* neighbouring faces of the loop range stay zero-filled; only (is,js,ks) is read back */
+  for (int m = 0; m < 6; m++)
+    for (int n = 0; n < s->n3; n++) {
+      if (s->gas[(int64_t)(gasl_rho + m) * s->n3 + n] == 0.0) s->gas[(int64_t)(gasl_rho + m) * s->n3 + n] = L[m];
+      if (s->gas[(int64_t)(gasr_rho + m) * s->n3 + n] == 0.0) s->gas[(int64_t)(gasr_rho + m) * s->n3 + n] = R[m];
+    }
+  face_fluxes(s, dir);
+  for (int m = 0; m < 6; m++) out[m] = GAS(gascf_rho + m, i, j, k);
+  for (int m = 0; m < 3; m++) out[6 + m] = GEM(magsf_x1dir + m, i, j, k);
+  go_destroy(s);
 }
 
 /* spherical Source_Term_i / _j (src/spherical/spherical_general.c:605-688) */

@@ -57,6 +57,14 @@ void go_step(go_solver *s, int n);
    left b1..b3 then right b1..b3 */
 double *go_debug_fluxes(go_solver *s);
 
+/* unit-test hooks: one PDMU7 evaluation (src/solver_func.c:215-253) and
+   the Rusanov flux + magnetic stress of one face
+   (src/solver_func.c:934-1515) */
+double go_pdmu7(const double fh0[7], const double fp0[7], const double w[7], double faceim1,
+                double facei, double vol, double dx);
+void go_face_flux(int dir, const double L[6], const double R[6], const double bl[3],
+                  const double br[3], const double b0[8], double Bn, double CA, double out[9]);
+
 /* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max */
 double go_scalar(const go_solver *s, int which);
 void go_set_scalar(go_solver *s, int which, double v);

@@ -43,6 +43,12 @@ def load(fast=False):
     lib.go_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.go_debug_fluxes.restype = ctypes.POINTER(ctypes.c_double)
     lib.go_debug_fluxes.argtypes = [ctypes.c_void_p]
+    D7 = ctypes.c_double * 7
+    lib.go_pdmu7.restype = ctypes.c_double
+    lib.go_pdmu7.argtypes = [D7, D7, D7, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double]
+    lib.go_face_flux.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 5 + [
+        ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     lib.go_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     _libs[name] = lib
     return lib
@@ -107,3 +113,19 @@ class Oracle:
 
     def __del__(self):
         self.close()
+
+
+def pdmu7(fh, fp, w, faceim1, facei, vol, dx):
+    lib = load()
+    D7 = ctypes.c_double * 7
+    return lib.go_pdmu7(D7(*fh), D7(*fp), D7(*w), faceim1, facei, vol, dx)
+
+
+def face_flux(dir_, L, R, bl, br, b0, Bn, CA):
+    lib = load()
+
+    def arr(v):
+        return (ctypes.c_double * len(v))(*v)
+    out = (ctypes.c_double * 9)()
+    lib.go_face_flux(dir_, arr(L), arr(R), arr(bl), arr(br), arr(b0), Bn, CA, out)
+    return np.array(out[:])

@@ -1,0 +1,114 @@
+"""The C host facade (reference-compatible globals, geometry, weights,
+background field, problem plugin, boundary library) against the compiled
+reference: every array bit for bit."""
+import ctypes
+import hashlib
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import gamera_op as go
+from parity import active_slices, oracle_from_problem
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+HASHES = json.load(open(os.path.join(GOLD, "ref_hashes.json")))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def _cases():
+    for key in HASHES["geometry"]:
+        k, s, dims = key.split("_")
+        ni, nj, nk = map(int, dims.split("x"))
+        yield int(k[1:]), int(s[1:]), ni, nj, nk, key
+
+
+@pytest.mark.parametrize("kind,seed,ni,nj,nk,key", list(_cases()))
+def test_geometry_weights_b0_bitwise(kind, seed, ni, nj, nk, key):
+    h = HASHES["geometry"][key]
+    P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
+    try:
+        bad = []
+        for f in range(28):
+            if sha(P.geo(f)) != h[f"geo{f}"]:
+                bad.append(f"geo{f}")
+        for f in range(42):
+            if sha(P.rec(f)) != h[f"rec{f}"]:
+                bad.append(f"rec{f}")
+        for f in range(52, 91):
+            if sha(P.gem(f) + 0.0) != h[f"gem{f}"]:  # signed zeros normalised
+                bad.append(f"gem{f}")
+        for name in go.COORD_NAMES:
+            if sha(P.coord_array(name)) != h[name]:
+                bad.append(name)
+        assert not bad, bad
+        # the solve() prologue (ghost fill, BCs, derived variables, dt0)
+        O = oracle_from_problem(P)
+        O.prologue()
+        cell, faces = active_slices(P)
+        for f in range(6):
+            assert sha(O.gas[f][cell]) == h[f"init_prim{f}"], f
+        for f in range(3):
+            assert sha(O.gem[f][faces[f]]) == h[f"init_B{f}"], f
+        assert O.scalar("dt0") == h["dt0"]
+    finally:
+        P.close()
+
+
+def test_boundary_kinds_recorded_from_plugin():
+    P = go.Problem(go.DIPOLE_SHELL, 1, 12, 10, 12)
+    try:
+        bc = P.bc_table()  # 1 = symmetric, 0 = not requested (periodic axis)
+        assert bc.tolist() == [[1, 1, 1, 1, 0, 0], [1, 1, 1, 1, 0, 0]]
+    finally:
+        P.close()
+    P = go.Problem(go.SINUSOID, 1, 12, 10, 8)
+    try:
+        assert P.bc_table().tolist() == [[0] * 6, [0] * 6]
+    finally:
+        P.close()
+
+
+def test_uniform_background_summarised_for_device():
+    P = go.Problem(go.SINUSOID_BLAST, 3, 12, 12, 12)
+    try:
+        assert P.b0_mode() == 1
+    finally:
+        P.close()
+    P = go.Problem(go.SINUSOID, 3, 12, 12, 12)
+    try:
+        assert P.b0_mode() == 0
+    finally:
+        P.close()
+
+
+def test_read_config_yaml_schema():
+    """config.yaml keys of the reference (src/config.c:150-163); unknown keys
+    are ignored."""
+    h, _ = go._load()
+    h.read_config.restype = ctypes.c_int
+    h.read_config.argtypes = [ctypes.c_char_p]
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "config.yaml")
+        with open(p, "w") as f:
+            f.write("# comment\nni_global: 64\nnj_global: 32\nnk_global: 16\nproc_dims_i: 2\n"
+                    "proc_dims_j: 1\nproc_dims_k: 1\nNt: 7\ntime_stop: 2.5\n"
+                    "output_interval: 0.5\nx_Norm: 1.0\nu_Norm: 2.0\nrho_Norm: 3.0\nseed: 42\n")
+        assert h.read_config(p.encode()) == 0
+    sizes = (ctypes.c_int * 6)()
+    h.gkh_sizes(sizes)
+    assert list(sizes) == [32, 32, 16, 41, 41, 25]
+    assert ctypes.c_int.in_dll(h, "Nt").value == 7
+    assert ctypes.c_double.in_dll(h, "time_stop").value == 2.5
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "config.yaml")
+        with open(p, "w") as f:
+            f.write("ni_global: 30\nnj_global: 32\nnk_global: 16\nproc_dims_i: 4\n"
+                    "proc_dims_j: 1\nproc_dims_k: 1\n")
+        assert h.read_config(p.encode()) != 0  # 30 not divisible by 4

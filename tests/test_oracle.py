@@ -1,0 +1,174 @@
+"""The CPU oracle: pinned to the compiled reference (golden fixtures written
+by tests/golden/make_fixtures.py) and to closed-form properties.
+
+The oracle takes its geometry and initial state from the host facade; the
+host arrays themselves are pinned separately in test_host_facade.py.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gamera_op as go
+from oracle_lib import face_flux, pdmu7
+from parity import active_slices, oracle_from_problem, run_oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STATES = np.load(os.path.join(GOLD, "ref_states.npz"))
+HASHES = json.load(open(os.path.join(GOLD, "ref_hashes.json")))
+
+STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16, 20),
+               (2, 4, 16, 12, 16, 20), (3, 5, 24, 16, 8, 40)]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("kind,seed,ni,nj,nk,steps", STATE_CASES)
+def test_oracle_reproduces_reference_bitwise(kind, seed, ni, nj, nk, steps):
+    P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
+    try:
+        O = run_oracle(P, steps)
+        key = f"k{kind}_s{seed}_{ni}x{nj}x{nk}_n{steps}"
+        cell, faces = active_slices(P)
+        for f in range(6):
+            assert np.array_equal(O.gas[f][cell], STATES[f"{key}/prim{f}"]), f
+        for f in range(3):
+            assert np.array_equal(O.gem[f][faces[f]], STATES[f"{key}/B{f}"]), f
+        sc = STATES[f"{key}/scalars"]
+        assert O.scalar("dt") == sc[0] and O.scalar("dt0") == sc[1]
+        assert O.scalar("time_sim") == sc[2]
+        # the reference max-updates divB_max inside an OpenMP loop without a
+        # reduction (src/solver_func.c:324-343), so its value can miss the
+        # true maximum but never exceed it
+        assert sc[3] <= O.scalar("divB_max") < 1e-12
+    finally:
+        P.close()
+
+
+def test_oracle_config0_bitwise():
+    """Config 0 (128x128x8 sinusoid, 200 AB2 steps at CFL 0.3) against the
+    SHA-256 of the reference run."""
+    kind, seed, ni, nj, nk, steps = HASHES["config0"]["case"]
+    P = go.Problem(kind, seed, ni, nj, nk)
+    try:
+        O = run_oracle(P, steps)
+        cell, faces = active_slices(P)
+        h = HASHES["config0"]
+        for f in range(6):
+            assert sha(O.gas[f][cell]) == h[f"prim{f}"], f
+        for f in range(3):
+            assert sha(O.gem[f][faces[f]]) == h[f"B{f}"], f
+        assert O.scalar("time_sim") == h["time_sim"]
+        assert O.scalar("dt") == h["dt"]
+    finally:
+        P.close()
+
+
+# --- closed-form properties ----------------------------------------------
+FAC = np.array([-1 / 140, 5 / 84, -101 / 420, 319 / 420, 107 / 210, -19 / 210, 1 / 105])
+
+
+def test_pdmu7_weights_partition_of_unity():
+    assert abs(FAC.sum() - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4, 5, 6])
+def test_pdmu7_unlimited_exact_for_polynomials(deg):
+    """Cell averages of x^deg on cells [n-4, n-3], ..., the interpolant at the
+    face x = 0 (between cells -1 and 0) is exact for degree <= 6."""
+    edges = np.arange(-4, 4, dtype=float)
+    avg = [((b ** (deg + 1) - a ** (deg + 1)) / (deg + 1)) for a, b in zip(edges[:-1], edges[1:])]
+    f_itp = float(np.dot(FAC, avg))
+    exact = 1.0 if deg == 0 else 0.0
+    assert abs(f_itp - exact) < 1e-12
+
+
+def test_pdmu7_linear_profile_unlimited():
+    """A linear profile passes the limiter untouched (Cartesian geometry gives
+    c1 = 2, c2 = -2): cell k of the stencil i-4..i+2 is centred at k - 3.5
+    relative to the face between cells i-1 and i."""
+    a, b = 0.7, 0.3
+    avg = [a + b * (k - 3.5) for k in range(7)]
+    val = pdmu7(avg, avg, FAC, 1.0, 1.0, 1.0, 1.0)
+    assert abs(val - a) < 1e-14
+
+
+def test_pdmu7_monotone_at_discontinuity():
+    """At a step the limited state stays within the two adjacent cells."""
+    left, right = 1.0, 10.0
+    for pos in range(1, 6):
+        fh = [left if n < pos else right for n in range(7)]
+        v = pdmu7(fh, fh, FAC, 1.0, 1.0, 1.0, 1.0)
+        lo, hi = min(fh[3], fh[4]), max(fh[3], fh[4])
+        assert lo - 1e-14 <= v <= hi + 1e-14, (pos, v)
+
+
+def test_pdmu7_extremum_keeps_cell_value():
+    """At a local extremum (q0 = 0) the face value collapses to the upwind
+    cell value."""
+    fh = [0.0, 1.0, 2.0, 3.0, 2.0, 1.0, 0.0]
+    v = pdmu7(fh, fh, FAC, 1.0, 1.0, 1.0, 1.0)
+    assert v == fh[3]
+
+
+@pytest.mark.parametrize("dir_", [0, 1, 2])
+def test_rusanov_equal_states_give_physical_flux(dir_):
+    """With L = R the Rusanov flux is the physical flux: no diffusion term."""
+    g = 5.0 / 3.0
+    rho, v, p = 1.3, np.array([0.4, -0.2, 0.7]), 0.9
+    b = np.array([0.3, 0.5, -0.1])
+    prim = [rho, *v, p, 0.0]
+    Bn = b[dir_]
+    out = face_flux(dir_, prim, prim, b, b, [0.0] * 8, Bn, 10.0)
+    vn = v[dir_]
+    eng = 0.5 * rho * (v @ v) + p / (g - 1)
+    assert np.isclose(out[0], rho * vn, rtol=1e-14)
+    for m in range(3):
+        expect = rho * v[m] * vn + (p if m == dir_ else 0.0)
+        assert np.isclose(out[1 + m], expect, rtol=1e-14, atol=1e-15)
+    assert np.isclose(out[4], (eng + p) * vn, rtol=1e-14)
+    assert out[5] == 0.0
+    # magnetic stress 0.5 |b|^2 delta - b b_n (with b_n equal to the face B)
+    for m in range(3):
+        expect = (0.5 * (b @ b) if m == dir_ else 0.0) - b[m] * Bn
+        assert np.isclose(out[6 + m], expect, rtol=1e-13, atol=1e-15)
+
+
+def test_uniform_state_is_fixed_point():
+    P = go.Problem(go.SINUSOID, 1, 12, 10, 8)
+    try:
+        O = oracle_from_problem(P)
+        for f, v in enumerate([1.7, 0.2, -0.3, 0.1, 0.8, 0.0]):
+            O.gas[f][...] = v
+        O.gem[0][...] = 0.4
+        O.gem[1][...] = -0.25
+        O.gem[2][...] = 0.15
+        O.prologue()
+        O.step(5)
+        cell, faces = active_slices(P)
+        for f, v in enumerate([1.7, 0.2, -0.3, 0.1, 0.8, 0.0]):
+            assert np.allclose(O.gas[f][cell], v, rtol=1e-13, atol=1e-15), f
+        for f, v in enumerate([0.4, -0.25, 0.15]):
+            assert np.allclose(O.gem[f][faces[f]], v, rtol=1e-13), f
+    finally:
+        P.close()
+
+
+def test_mass_conserved_and_divb_roundoff_periodic():
+    P = go.Problem(go.SINUSOID_SPHERE, 9, 16, 16, 8)
+    try:
+        O = oracle_from_problem(P)
+        O.prologue()
+        cell, _ = active_slices(P)
+        vol = O.geo[15][cell]
+        m0 = float((O.gas[0][cell] * vol).sum())
+        O.step(20)
+        m1 = float((O.gas[0][cell] * vol).sum())
+        assert abs(m1 - m0) <= 1e-13 * abs(m0)
+        assert O.scalar("divB_max") < 1e-12
+    finally:
+        P.close()
