@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the per-step ideal-MHD update on one MI355X.
+
+Workload (default): Cartesian periodic box [0,1]^3 at 256^3 cells, seeded
+sinusoidal velocity and edge-potential field, uniform |B0| = 1 background in
+a seeded direction and an over-pressure sphere (R = 0.1, p ratio 100); AB2
+steps at CFL 0.3, FP64 arithmetic.  One "step" is one full AB2 step over the
+grid (all stage kernels, dt reduction included); inputs are resident in HBM
+when the timed region starts.
+
+Prints ONE JSON line with the measured cell-updates/s, the roofline of the
+dominant kernel (HIP events on the solver's stream over the timed region) and
+a CPU baseline (the CPU restatement of the same step, timed on this host's
+cores on a bounded sample).
+
+  python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gamera-op_amd")
+HBM_PEAK_GBS = 8000.0
+GAMMA = 5.0 / 3.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_alg_bytes(name, n):
+    """Compulsory HBM bytes of one launch of a stage kernel on an n^3 grid
+    (each distinct input value read once, each output written once, 8 B per
+    value).  Counts follow the kernel's own inputs and outputs."""
+    N = n + 9
+    full_faces = (n + 1) * n * n
+    if name.startswith("k_sweep_"):
+        # full faces: X^n, X^{n-1} (6+6), cell B at n, n-1 (3+3), face-normal
+        # B (1) in; 9 fluxes + 2 face velocities out.  The transverse
+        # extension (8 extra rows) only forms the 2 face velocities: 2+2 in,
+        # 2 out.
+        ext_faces = (n + 1) * (N - 1 - n) * n
+        return 8 * (full_faces * 30 + ext_faces * 6)
+    if name.startswith("k_emf_"):
+        # per edge: 2 face velocities, 2 face-B (n, n-1 each), density at n, n-1
+        # in, 1 EMF out
+        return 8 * n * (n + 1) * (n + 1) * 9
+    if name == "k_ct":
+        # 3 face-B in, 3 EMF in, 3 face-B out
+        return 8 * 9 * full_faces
+    if name == "k_cell":
+        # 27 face fluxes, X^n (6), cell B^n (3), face B^{n+1} (3) in; X^{n+1} out
+        return 8 * n ** 3 * 45
+    if name == "k_ghost":
+        return 8 * 2 * 9 * (N ** 3 - n ** 3)
+    if name == "k_mags":
+        return 8 * 6 * (N - 1) ** 3
+    return 0
+
+
+def run_gpu(args):
+    sys.path.insert(0, PKG)
+    import gamera_op as go
+
+    n = args.n
+    log(f"bench: host setup {n}^3 (sinusoid + uniform B0 + sphere, seed {args.seed})")
+    t0 = time.time()
+    P = go.Problem(go.SINUSOID_BLAST, args.seed, n, n, n)
+    log(f"bench: host setup {time.time() - t0:.1f} s; device open")
+    P.device_open(0)
+    log(f"bench: device holds {P.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
+    P.device_step(args.warmup)
+    P.enable_timing(True)
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    P.device_step(args.steps)  # returns after the solver's stream has drained
+    el = time.perf_counter() - t0
+    kt = P.kernel_times()  # average ms per launch over the timed steps
+    P.enable_timing(False)
+    sc = P.device_scalars()
+    log(f"bench: {el * 1e3 / args.steps:.3f} ms/step; scalars {sc}")
+    if sc["nonfinite"]:
+        raise RuntimeError("non-finite values in the solution")
+    log("bench: copy calibration")
+    cbytes = 2 * 8 * (n + 9) ** 3 * 6  # read + write of six fields
+    cms = go.copy_calibrate(cbytes // 2, 20)
+    P.close()
+    return el, kt, sc, cbytes / (cms * 1e-3) / 1e9
+
+
+def run_cpu(args):
+    """CPU restatement on a bounded sample of the same workload."""
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, PKG)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import gamera_op as go
+    from parity import oracle_from_problem
+
+    n, steps = args.cpu_n, args.cpu_steps
+    log(f"bench: CPU sample {n}^3, {steps} steps, {threads} threads")
+    P = go.Problem(go.SINUSOID_BLAST, args.seed, n, n, n, full_b0=True)
+    O = oracle_from_problem(P, fast=True)
+    P.close()
+    O.prologue()
+    O.step(1)  # warm-up
+    t0 = time.perf_counter()
+    O.step(steps)
+    el = time.perf_counter() - t0
+    log(f"bench: CPU {el:.2f} s")
+    return {"value": n ** 3 * steps / el, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n}^3 cells of the same generator, {steps} AB2 steps after 1 warm-up, "
+                      f"oracle/liboracle_fast.so (-O3 -march=native, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the solver runs a single-GPU domain "
+              "(multi-GPU sharding is not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+
+    el, kt, sc, copy_gbs = run_gpu(args)
+    n = args.n
+    cells = n ** 3
+    value = cells * args.steps / el
+    ms_step = el * 1e3 / args.steps
+    # dominant kernel: largest time per step (each kernel launches once per step)
+    dom = max(kt, key=lambda k: kt[k])
+    dom_ms = kt[dom]
+    alg = kernel_alg_bytes(dom, n)
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"{dom}_{n}cube_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fp:
+            traffic = json.load(fp).get("bytes_per_launch")
+    cpu = None if args.no_cpu else run_cpu(args)
+    step_ms_kernels = sum(kt.values())
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"cartesian periodic {n}^3, sinusoid + uniform B0 |B|=1 + "
+                               f"over-pressure sphere, AB2 CFL 0.3, FP64",
+                   "cells": cells, "seed": args.seed, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": dom_ms},
+        "step_roofline": {"alg_bytes_per_cell_update": 216,
+                          "achieved_GBs": 216 * value / 1e9,
+                          "frac": 216 * value / 1e9 / HBM_PEAK_GBS,
+                          "kernel_ms_per_step": step_ms_kernels},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+        "copy_kernel_GBs": copy_gbs,
+        "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -761,6 +761,14 @@ __global__ __launch_bounds__(256) void k_mags(MagsArgs a) {
     a.M[m * g.fs + c] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
 }
 
+// plain copy, 16 bytes per lane per iteration (calibration kernel)
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    dst[q] = src[q];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -790,6 +798,8 @@ struct gk_solver {
   std::vector<std::string> names;
   std::vector<double> acc_ms;
   std::vector<int> acc_n;
+  std::vector<hipEvent_t> evpool;  // timing events, reused
+  std::vector<int> pend;           // kernel slot per recorded event pair
 };
 
 #define HIPCHK(x)                                                              \
@@ -957,6 +967,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   }
   (void)hipFree(s->E);
   (void)hipFree(s->sc);
+  for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return GK_OK;
@@ -1076,11 +1087,12 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
 }
 
 namespace {
-// per-kernel timing with events on the solver's stream (only when enabled)
+// Per-kernel timing with events on the solver's stream (only when enabled).
+// Events come from a pool and are resolved after the stream synchronises at
+// the end of gk_step, so timing does not serialise the host with the GPU.
 struct Timer {
   gk_solver *s;
   int slot;
-  hipEvent_t e0, e1;
   Timer(gk_solver *s_, const char *name) : s(s_), slot(-1) {
     if (!s->timing) return;
     for (size_t n = 0; n < s->names.size(); n++)
@@ -1091,22 +1103,35 @@ struct Timer {
       s->acc_ms.push_back(0.0);
       s->acc_n.push_back(0);
     }
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventRecord(e0, s->stream);
+    const size_t need = 2 * (s->pend.size() + 1);
+    while (s->evpool.size() < need) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        slot = -1;
+        return;
+      }
+      s->evpool.push_back(e);
+    }
+    (void)hipEventRecord(s->evpool[2 * s->pend.size()], s->stream);
   }
   ~Timer() {
     if (slot < 0) return;
-    (void)hipEventRecord(e1, s->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    s->acc_ms[slot] += ms;
-    s->acc_n[slot] += 1;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipEventRecord(s->evpool[2 * s->pend.size() + 1], s->stream);
+    s->pend.push_back(slot);
   }
 };
+
+// accumulate the recorded kernel times (stream already synchronised)
+void resolve_timers(gk_solver *s) {
+  for (size_t n = 0; n < s->pend.size(); n++) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, s->evpool[2 * n], s->evpool[2 * n + 1]) == hipSuccess) {
+      s->acc_ms[s->pend[n]] += ms;
+      s->acc_n[s->pend[n]] += 1;
+    }
+  }
+  s->pend.clear();
+}
 }  // namespace
 
 static void one_step(gk_solver *s) {
@@ -1234,9 +1259,16 @@ extern "C" int gk_step(gk_solver *s, int n) {
       s->acc_ms[q] = 0.0;
       s->acc_n[q] = 0;
     }
-  for (int t = 0; t < n; t++) one_step(s);
+  for (int t = 0; t < n; t++) {
+    one_step(s);
+    if (s->timing && s->pend.size() >= 4096) {  // bound the event pool
+      HIPCHK(hipStreamSynchronize(s->stream));
+      resolve_timers(s);
+    }
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->timing) resolve_timers(s);
   return GK_OK;
 }
 
@@ -1312,4 +1344,37 @@ extern "C" int gk_kernel_times(gk_solver *s, const char **names, double *ms, int
   }
   *n = cnt;
   return GK_OK;
+}
+
+extern "C" int gk_copy_calibrate(size_t bytes, int reps, double *ms) {
+  if (!ms || reps < 1 || bytes < 16) return GK_ERR_ARG;
+  const size_t n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = GK_OK;
+  if (hipMalloc((void **)&a, n * 16) != hipSuccess || hipMalloc((void **)&b, n * 16) != hipSuccess) {
+    rc = GK_ERR_NOMEM;
+  } else if (hipMemset(a, 0, n * 16) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+             hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = GK_ERR_DEVICE;
+  } else {
+    const unsigned blocks = 256 * 32;  // 32 workgroups per CU over 256 CUs
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a, b, n);
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a, b, n);
+    (void)hipEventRecord(e1, st);
+    float t = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      rc = GK_ERR_DEVICE;
+    *ms = (double)t / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }

@@ -67,6 +67,8 @@ def _load():
                                    ctypes.POINTER(ctypes.c_double)]
     _op.gk_device_bytes.restype = ctypes.c_size_t
     _op.gk_device_bytes.argtypes = [ctypes.c_void_p]
+    _op.gk_copy_calibrate.restype = ctypes.c_int
+    _op.gk_copy_calibrate.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _host = h
     return _host, _op
 
@@ -76,6 +78,17 @@ def exported_symbols():
     return ["gk_create", "gk_destroy", "gk_set_state", "gk_prologue", "gk_step",
             "gk_get_state", "gk_get_scalars", "gk_set_time", "gk_enable_timing",
             "gk_kernel_times", "gk_device_bytes"]
+
+
+def copy_calibrate(nbytes, reps=20):
+    """Average duration (ms) of a plain device copy of nbytes (read nbytes,
+    write nbytes)."""
+    _, op = _load()
+    ms = ctypes.c_double(0.0)
+    rc = op.gk_copy_calibrate(nbytes, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise GkError(f"gk_copy_calibrate failed ({rc})")
+    return ms.value
 
 
 class GkError(RuntimeError):

@@ -124,6 +124,12 @@ int gk_debug_fetch(gk_solver *s, int which, int field, double *host);
 /* device bytes held by the solver */
 size_t gk_device_bytes(const gk_solver *s);
 
+/* Measurement helper (no counterpart in the reference): a plain device copy
+   of `bytes` bytes (rounded down to 16-byte units), `reps` times after one
+   warm-up; *ms receives the average duration of one copy kernel.  Used to
+   measure the achievable HBM bandwidth and to scale hardware byte counters. */
+int gk_copy_calibrate(size_t bytes, int reps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
