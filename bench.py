@@ -71,6 +71,8 @@ def run_gpu(args):
     P = go.Problem(go.SINUSOID_BLAST, args.seed, n, n, n)
     log(f"bench: host setup {time.time() - t0:.1f} s; device open")
     P.device_open(0)
+    if args.chunk:
+        P.set_tuning(go.TUNE_CHUNK, args.chunk)
     log(f"bench: device holds {P.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
     P.device_step(args.warmup)
     P.enable_timing(True)
@@ -128,6 +130,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0, help="faces per thread in the i/j sweeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs a single-GPU domain "

@@ -231,6 +231,71 @@ __global__ void k_set_dt0(Scal *sc) {
   sc->dt0 = __longlong_as_double((long long)sc->dtmin);
 }
 
+// Face flux from the reconstructed states: Rusanov hydro flux (reference
+// src/solver_func.c:934-1251) and magnetic stress with B0 cross terms and
+// Boris diffusion (src/solver_func.c:1268-1515).  L/R: rho, v1..3, p, p_S;
+// bl/br: cell-centred B (without B0); Bn: face-normal B at the face.
+// Fo: mass, momentum 1..3, energy, entropy, magnetic stress 1..3.
+template <int D>
+__device__ __forceinline__ void face_flux(const double L[6], const double R[6], const double bl[3],
+                                          const double br[3], const double Bn, const B0c &b0c,
+                                          const double CA, double Fo[9]) {
+  const double *b0 = b0c.face[D];
+  const double rl = L[0], rr = R[0], pl = L[4], pr = R[4], psl = L[5], psr = R[5];
+  const double vl[3] = {L[1], L[2], L[3]}, vr[3] = {R[1], R[2], R[3]};
+  double csl = sqrt(GAMMA * pl / rl), csr = sqrt(GAMMA * pr / rr);
+  double cel = sqrt(GAMMA * psl / rl), cer = sqrt(GAMMA * psr / rr);
+  double Btl = (bl[0] + b0[0]) * (bl[0] + b0[0]) + (bl[1] + b0[1]) * (bl[1] + b0[1]) +
+               (bl[2] + b0[2]) * (bl[2] + b0[2]);
+  double Btr = (br[0] + b0[0]) * (br[0] + b0[0]) + (br[1] + b0[1]) * (br[1] + b0[1]) +
+               (br[2] + b0[2]) * (br[2] + b0[2]);
+  double Val = sqrt(Btl / rl), Var = sqrt(Btr / rr);
+  double Vel = Val * CA / (sqrt(Val * Val + CA * CA));
+  double Ver = Var * CA / (sqrt(Var * Var + CA * CA));
+  double vnl = vl[D], vnr = vr[D];
+  double VF = fmax(fabs(vnl) + sqrt(csl * csl + Vel * Vel), fabs(vnr) + sqrt(csr * csr + Ver * Ver));
+  double VFe = fmax(fabs(vnl) + sqrt(cel * cel + Vel * Vel), fabs(vnr) + sqrt(cer * cer + Ver * Ver));
+  double rvl[3], rvr[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    rvl[m] = rl * vl[m];
+    rvr[m] = rr * vr[m];
+  }
+  double engl = 0.5 * rl * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) + pl / (GAMMA - 1.0);
+  double engr = 0.5 * rr * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) + pr / (GAMMA - 1.0);
+  double Sl = entropy_over(psl, rl), Sr = entropy_over(psr, rr);
+  Fo[0] = 0.5 * (rvl[D] + rvr[D] - VF * (rr - rl));
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    if (m == D)
+      Fo[1 + m] = 0.5 * (rvl[m] * vnl + pl + rvr[m] * vnr + pr - VF * (rvr[m] - rvl[m]));
+    else
+      Fo[1 + m] = 0.5 * (rvl[m] * vnl + rvr[m] * vnr - VF * (rvr[m] - rvl[m]));
+  }
+  Fo[4] = 0.5 * (engl * vnl + pl * vnl + engr * vnr + pr * vnr - VF * (engr - engl));
+  Fo[5] = 0.5 * (Sl * vnl + Sr * vnr - VFe * (Sr - Sl));
+
+  const double b0n = b0[3];
+  double bsql = (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]) +
+                2 * (bl[0] * b0[0] + bl[1] * b0[1] + bl[2] * b0[2]);
+  double bsqr = (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]) +
+                2 * (br[0] * b0[0] + br[1] * b0[1] + br[2] * b0[2]);
+  double Bsq = (Btl + Btr) / 4 / CA;
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double sp, sn;
+    if (m == D) {
+      sp = 0.5 * (0.5 * bsql - (bl[m] + b0[m]) * Bn - bl[m] * b0n);
+      sn = 0.5 * (0.5 * bsqr - (br[m] + b0[m]) * Bn - br[m] * b0n);
+    } else {
+      sp = 0.5 * (-(bl[m] + b0[m]) * Bn - bl[m] * b0n);
+      sn = 0.5 * (-(br[m] + b0[m]) * Bn - br[m] * b0n);
+    }
+    double ms = sp + sn;
+    Fo[6 + m] = ms - Bsq * (vr[m] - vl[m]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 struct SweepArgs {
   Grid g;
@@ -279,68 +344,207 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
 #pragma unroll
   for (int m = 0; m < 3; m++) recon(a.M + m * g.fs, a.Mp + m * g.fs, c, sd, cc, T, bl[m], br[m]);
 
-  // Rusanov hydro flux (reference src/solver_func.c:934-1251)
-  const double CA = g.CA;
-  const double *b0 = a.b0.face[D];
-  const double rl = L[0], rr = R[0], pl = L[4], pr = R[4], psl = L[5], psr = R[5];
-  const double vl[3] = {L[1], L[2], L[3]}, vr[3] = {R[1], R[2], R[3]};
-  double csl = sqrt(GAMMA * pl / rl), csr = sqrt(GAMMA * pr / rr);
-  double cel = sqrt(GAMMA * psl / rl), cer = sqrt(GAMMA * psr / rr);
-  double Btl = (bl[0] + b0[0]) * (bl[0] + b0[0]) + (bl[1] + b0[1]) * (bl[1] + b0[1]) +
-               (bl[2] + b0[2]) * (bl[2] + b0[2]);
-  double Btr = (br[0] + b0[0]) * (br[0] + b0[0]) + (br[1] + b0[1]) * (br[1] + b0[1]) +
-               (br[2] + b0[2]) * (br[2] + b0[2]);
-  double Val = sqrt(Btl / rl), Var = sqrt(Btr / rr);
-  double Vel = Val * CA / (sqrt(Val * Val + CA * CA));
-  double Ver = Var * CA / (sqrt(Var * Var + CA * CA));
-  double vnl = vl[D], vnr = vr[D];
-  double VF = fmax(fabs(vnl) + sqrt(csl * csl + Vel * Vel), fabs(vnr) + sqrt(csr * csr + Ver * Ver));
-  double VFe = fmax(fabs(vnl) + sqrt(cel * cel + Vel * Vel), fabs(vnr) + sqrt(cer * cer + Ver * Ver));
-  double rvl[3], rvr[3];
+  double Fo[9];
+  face_flux<D>(L, R, bl, br, a.B[D * g.fs + c], a.b0, g.CA, Fo);
 #pragma unroll
-  for (int m = 0; m < 3; m++) {
-    rvl[m] = rl * vl[m];
-    rvr[m] = rr * vr[m];
-  }
-  double engl = 0.5 * rl * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) + pl / (GAMMA - 1.0);
-  double engr = 0.5 * rr * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) + pr / (GAMMA - 1.0);
-  double Sl = entropy_over(psl, rl), Sr = entropy_over(psr, rr);
-  double *F = a.F;
-  F[c] = 0.5 * (rvl[D] + rvr[D] - VF * (rr - rl));
-#pragma unroll
-  for (int m = 0; m < 3; m++) {
-    double fl;
-    if (m == D)
-      fl = 0.5 * (rvl[m] * vnl + pl + rvr[m] * vnr + pr - VF * (rvr[m] - rvl[m]));
-    else
-      fl = 0.5 * (rvl[m] * vnl + rvr[m] * vnr - VF * (rvr[m] - rvl[m]));
-    F[(1 + m) * g.fs + c] = fl;
-  }
-  F[4 * g.fs + c] = 0.5 * (engl * vnl + pl * vnl + engr * vnr + pr * vnr - VF * (engr - engl));
-  F[5 * g.fs + c] = 0.5 * (Sl * vnl + Sr * vnr - VFe * (Sr - Sl));
+  for (int m = 0; m < 9; m++) a.F[m * g.fs + c] = Fo[m];
+}
 
-  // magnetic stress with B0 cross terms and Boris diffusion
-  // (reference src/solver_func.c:1268-1515)
-  const double Bn = a.B[D * g.fs + c];
-  const double b0n = b0[3];
-  double bsql = (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]) +
-                2 * (bl[0] * b0[0] + bl[1] * b0[1] + bl[2] * b0[2]);
-  double bsqr = (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]) +
-                2 * (br[0] * b0[0] + br[1] * b0[1] + br[2] * b0[2]);
-  double Bsq = (Btl + Btr) / 4 / CA;
+// Marching sweep along D = 0 (i) or 1 (j).  Lanes run along k (coalesced),
+// block rows along the other transverse axis RA, and each thread walks a
+// chunk of faces along D, keeping for every reconstructed field a register
+// window of the 8 half-step values and the 6 current values the stencil and
+// limiter need: each face loads one new cell instead of eight.  Rows outside
+// [flo, fhi] of RA only form the two face-averaged velocities.  The per-face
+// arithmetic is that of k_sweep.
+template <int NF>
+struct Window {
+  double h[NF][8];  // half-step values at cells f-4 .. f+3
+  double x[NF][6];  // current values at cells f-2 .. f+3
+};
+
+template <int D, int NF>
+__device__ __forceinline__ void march_load(Window<NF> &w, const double *const P[NF],
+                                           const double *const Q[NF], long o, double cc, int slot) {
 #pragma unroll
-  for (int m = 0; m < 3; m++) {
-    double sp, sn;
-    if (m == D) {
-      sp = 0.5 * (0.5 * bsql - (bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = 0.5 * (0.5 * bsqr - (br[m] + b0[m]) * Bn - br[m] * b0n);
-    } else {
-      sp = 0.5 * (-(bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = 0.5 * (-(br[m] + b0[m]) * Bn - br[m] * b0n);
-    }
-    double ms = sp + sn;
-    F[(6 + m) * g.fs + c] = ms - Bsq * (vr[m] - vl[m]);
+  for (int m = 0; m < NF; m++) {
+    const double x = P[m][o];
+    w.h[m][slot] = x + cc * (x - Q[m][o]);
+    if (slot >= 2) w.x[m][slot - 2] = x;
   }
+}
+
+template <int NF>
+__device__ __forceinline__ void march_shift(Window<NF> &w) {
+#pragma unroll
+  for (int m = 0; m < NF; m++) {
+#pragma unroll
+    for (int n = 0; n < 7; n++) w.h[m][n] = w.h[m][n + 1];
+#pragma unroll
+    for (int n = 0; n < 5; n++) w.x[m][n] = w.x[m][n + 1];
+  }
+}
+
+template <int NF>
+__device__ __forceinline__ void march_lr(const Window<NF> &w, int m, const double *T, double &L,
+                                         double &R) {
+  double fl = 0.0, fr = 0.0;
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    fl += w.h[m][n] * T[n];
+    fr += w.h[m][7 - n] * T[7 + n];
+  }
+  // limiter cells: x[0] = f-2, x[1] = f-1, x[2] = f, x[3] = f+1
+  L = pdm_limit(fl, w.x[m][0], w.x[m][1], w.x[m][2], T[14], T[15]);
+  R = pdm_limit(fr, w.x[m][3], w.x[m][2], w.x[m][1], T[16], T[17]);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
+  constexpr int RA = D == 0 ? 1 : 0;  // row axis
+  const Grid &g = a.g;
+  const int k = a.lo[2] + blockIdx.x * 64 + threadIdx.x;
+  const int r = a.lo[RA] + blockIdx.y * blockDim.y + threadIdx.y;
+  if (k > a.hi[2] || r > a.hi[RA]) return;
+  const int f0 = a.lo[D] + blockIdx.z * chunk;
+  const int f1 = min(f0 + chunk, a.hi[D] + 1);
+  int idx[3];
+  idx[2] = k;
+  idx[RA] = r;
+  idx[D] = f0;
+  const long sd = g.st[D];
+  long c = idx[0] * g.st[0] + idx[1] * g.st[1] + k;
+  const double cc = a.sc->c;
+  const double *T = rtab<D>(a.geo, f0, idx);
+  const long tstep = (long)a.geo.ncls[D][0] * a.geo.ncls[D][1] * GK_RECW_STRIDE;
+  constexpr int va = D == 0 ? 1 : 2;
+  constexpr int vb = D == 0 ? 2 : 3;
+  const long fs = g.fs;
+  if (r >= a.flo[RA] && r <= a.fhi[RA]) {
+    const double *P[9], *Q[9];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      P[m] = a.X + m * fs;
+      Q[m] = a.Xp + m * fs;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      P[6 + m] = a.M + m * fs;
+      Q[6 + m] = a.Mp + m * fs;
+    }
+    Window<9> w;
+#pragma unroll
+    for (int p = 0; p < 7; p++) march_load<D, 9>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int f = f0; f < f1; f++) {
+      march_load<D, 9>(w, P, Q, c + 3 * sd, cc, 7);
+      double L[6], R[6], bl[3], br[3];
+#pragma unroll
+      for (int m = 0; m < 6; m++) march_lr(w, m, T, L[m], R[m]);
+#pragma unroll
+      for (int m = 0; m < 3; m++) march_lr(w, 6 + m, T, bl[m], br[m]);
+      a.V[c] = 0.5 * (L[va] + R[va]);
+      a.V[fs + c] = 0.5 * (L[vb] + R[vb]);
+      double Fo[9];
+      face_flux<D>(L, R, bl, br, a.B[D * fs + c], a.b0, g.CA, Fo);
+#pragma unroll
+      for (int m = 0; m < 9; m++) a.F[m * fs + c] = Fo[m];
+      march_shift(w);
+      c += sd;
+      T += tstep;
+    }
+  } else {
+    const double *P[2] = {a.X + va * fs, a.X + vb * fs};
+    const double *Q[2] = {a.Xp + va * fs, a.Xp + vb * fs};
+    Window<2> w;
+#pragma unroll
+    for (int p = 0; p < 7; p++) march_load<D, 2>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int f = f0; f < f1; f++) {
+      march_load<D, 2>(w, P, Q, c + 3 * sd, cc, 7);
+      double l, rr;
+      march_lr(w, 0, T, l, rr);
+      a.V[c] = 0.5 * (l + rr);
+      march_lr(w, 1, T, l, rr);
+      a.V[fs + c] = 0.5 * (l + rr);
+      march_shift(w);
+      c += sd;
+      T += tstep;
+    }
+  }
+}
+
+// k sweep (D = 2) with the stencil staged in LDS: a block covers 64 faces
+// along k times KROWS rows along j at one i; each of the reconstructed fields
+// is loaded once per block (72 cells per row: 64 faces + 8 stencil halo) as
+// half-step and current values.  Planes i outside [flo0, fhi0] only form the
+// face-averaged velocities (v3, v1).
+constexpr int KROWS = 4;
+constexpr int KSPAN = 72;
+
+__global__ __launch_bounds__(256) void k_sweepk(SweepArgs a) {
+  __shared__ double sh[9][KROWS][KSPAN];
+  __shared__ double sx[9][KROWS][KSPAN];
+  const Grid &g = a.g;
+  const int k0 = a.lo[2] + blockIdx.x * 64;
+  const int j0 = a.lo[1] + blockIdx.y * KROWS;
+  const int i = a.lo[0] + blockIdx.z;
+  const long fs = g.fs;
+  const bool full = i >= a.flo[0] && i <= a.fhi[0];
+  const double cc = a.sc->c;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  // stage: points k0-4 .. k0+67 of rows j0 .. j0+KROWS-1 (clamped to the box)
+  const int nf = full ? 9 : 2;
+  for (int q = tid; q < nf * KROWS * KSPAN; q += 256) {
+    const int m = q / (KROWS * KSPAN);
+    const int rr = (q / KSPAN) % KROWS;
+    const int kk = q % KSPAN;
+    const int field = full ? m : (m == 0 ? 3 : 1);
+    const int slot = full ? m : (m == 0 ? 3 : 1);
+    const int j = min(j0 + rr, g.N[1] - 1);
+    const int kx = min(k0 - 4 + kk, g.N[2] - 1);
+    const long o = i * g.st[0] + j * g.st[1] + kx;
+    const double *P = field < 6 ? a.X + field * fs : a.M + (field - 6) * fs;
+    const double *Q = field < 6 ? a.Xp + field * fs : a.Mp + (field - 6) * fs;
+    const double x = P[o];
+    sh[slot][rr][kk] = x + cc * (x - Q[o]);
+    sx[slot][rr][kk] = x;
+  }
+  __syncthreads();
+  const int k = k0 + threadIdx.x;
+  const int j = j0 + threadIdx.y;
+  if (k > a.hi[2] || j > a.hi[1]) return;
+  int idx[3] = {i, j, k};
+  const double *T = rtab<2>(a.geo, k, idx);
+  const long c = i * g.st[0] + j * g.st[1] + k;
+  const int ry = threadIdx.y, b = threadIdx.x;  // stencil cells b .. b+7 in LDS
+  auto lr = [&](int m, double &L, double &R) {
+    double fl = 0.0, fr = 0.0;
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+      fl += sh[m][ry][b + n] * T[n];
+      fr += sh[m][ry][b + 7 - n] * T[7 + n];
+    }
+    L = pdm_limit(fl, sx[m][ry][b + 2], sx[m][ry][b + 3], sx[m][ry][b + 4], T[14], T[15]);
+    R = pdm_limit(fr, sx[m][ry][b + 5], sx[m][ry][b + 4], sx[m][ry][b + 3], T[16], T[17]);
+  };
+  if (!full) {
+    double l, r;
+    lr(3, l, r);
+    a.V[c] = 0.5 * (l + r);
+    lr(1, l, r);
+    a.V[fs + c] = 0.5 * (l + r);
+    return;
+  }
+  double L[6], R[6], bl[3], br[3];
+#pragma unroll
+  for (int m = 0; m < 6; m++) lr(m, L[m], R[m]);
+#pragma unroll
+  for (int m = 0; m < 3; m++) lr(6 + m, bl[m], br[m]);
+  a.V[c] = 0.5 * (L[3] + R[3]);
+  a.V[fs + c] = 0.5 * (L[1] + R[1]);
+  double Fo[9];
+  face_flux<2>(L, R, bl, br, a.B[2 * fs + c], a.b0, g.CA, Fo);
+#pragma unroll
+  for (int m = 0; m < 9; m++) a.F[m * fs + c] = Fo[m];
 }
 
 // ---------------------------------------------------------------------------
@@ -795,6 +999,7 @@ struct gk_solver {
   int ready;
   size_t bytes;
   int timing;
+  int chunk;           // faces per thread in the marching sweeps
   std::vector<std::string> names;
   std::vector<double> acc_ms;
   std::vector<int> acc_n;
@@ -943,6 +1148,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(GK_ERR_DEVICE);
   if (hipDeviceSynchronize() != hipSuccess) return fail(GK_ERR_DEVICE);
+  s->chunk = 32;
   s->cur = 0;
   s->b_m1 = 0;
   s->b_0 = 1;
@@ -1170,13 +1376,33 @@ static void one_step(gk_solver *s) {
     const int X2 = D == 0 ? 1 : (D == 1 ? 2 : 0);
     a.lo[X2] = 0;
     a.hi[X2] = g.N[X2] - 2;
-    dim3 grid(((a.hi[2] - a.lo[2] + 1) + 63) / 64, ((a.hi[1] - a.lo[1] + 1) + 3) / 4,
-              a.hi[0] - a.lo[0] + 1);
     static const char *nm[3] = {"k_sweep_i", "k_sweep_j", "k_sweep_k"};
     Timer t(s, nm[D]);
-    if (D == 0) hipLaunchKernelGGL(k_sweep<0>, grid, dim3(64, 4), 0, s->stream, a);
-    if (D == 1) hipLaunchKernelGGL(k_sweep<1>, grid, dim3(64, 4), 0, s->stream, a);
-    if (D == 2) hipLaunchKernelGGL(k_sweep<2>, grid, dim3(64, 4), 0, s->stream, a);
+    const int ch = s->chunk;
+    if (D == 0) {
+      dim3 grid((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + 4) / 4,
+                (a.hi[0] - a.lo[0] + ch) / ch);
+      hipLaunchKernelGGL(k_march<0>, grid, dim3(64, 4), 0, s->stream, a, ch);
+    } else if (D == 1) {
+      SweepArgs m = a;  // full faces: k over the active range
+      m.lo[2] = S[2];
+      m.hi[2] = Ee[2];
+      dim3 grid((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[0] - m.lo[0] + 4) / 4,
+                (m.hi[1] - m.lo[1] + ch) / ch);
+      hipLaunchKernelGGL(k_march<1>, grid, dim3(64, 4), 0, s->stream, m, ch);
+      // velocity-only strips k in [0, ks-1] and [ke+1, NK-2]
+      for (int side = 0; side < 2; side++) {
+        SweepArgs v = a;
+        v.lo[2] = side == 0 ? 0 : Ee[2] + 1;
+        v.hi[2] = side == 0 ? S[2] - 1 : g.N[2] - 2;
+        dim3 gv((v.hi[2] - v.lo[2] + 64) / 64, (v.hi[1] - v.lo[1] + 4) / 4, v.hi[0] - v.lo[0] + 1);
+        hipLaunchKernelGGL(k_sweep<1>, gv, dim3(64, 4), 0, s->stream, v);
+      }
+    } else {
+      dim3 grid((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + KROWS) / KROWS,
+                a.hi[0] - a.lo[0] + 1);
+      hipLaunchKernelGGL(k_sweepk, grid, dim3(64, KROWS), 0, s->stream, a);
+    }
   }
   // edge electric fields
   for (int D = 0; D < 3; D++) {
@@ -1377,4 +1603,20 @@ extern "C" int gk_copy_calibrate(size_t bytes, int reps, double *ms) {
   (void)hipFree(a);
   (void)hipFree(b);
   return rc;
+}
+
+extern "C" int gk_set_tuning(gk_solver *s, int param, int value) {
+  if (!s) return GK_ERR_ARG;
+  if (param == GK_TUNE_CHUNK) {
+    if (value < 1 || value > 4096) return GK_ERR_ARG;
+    s->chunk = value;
+    return GK_OK;
+  }
+  return GK_ERR_ARG;
+}
+
+extern "C" int gk_get_tuning(const gk_solver *s, int param) {
+  if (!s) return GK_ERR_ARG;
+  if (param == GK_TUNE_CHUNK) return s->chunk;
+  return GK_ERR_ARG;
 }

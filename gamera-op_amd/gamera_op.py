@@ -67,6 +67,10 @@ def _load():
                                    ctypes.POINTER(ctypes.c_double)]
     _op.gk_device_bytes.restype = ctypes.c_size_t
     _op.gk_device_bytes.argtypes = [ctypes.c_void_p]
+    _op.gk_set_tuning.restype = ctypes.c_int
+    _op.gk_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    _op.gk_get_tuning.restype = ctypes.c_int
+    _op.gk_get_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int]
     _op.gk_copy_calibrate.restype = ctypes.c_int
     _op.gk_copy_calibrate.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _host = h
@@ -89,6 +93,9 @@ def copy_calibrate(nbytes, reps=20):
     if rc != 0:
         raise GkError(f"gk_copy_calibrate failed ({rc})")
     return ms.value
+
+
+TUNE_CHUNK = 0
 
 
 class GkError(RuntimeError):
@@ -183,6 +190,14 @@ class Problem:
         if rc != 0:
             raise GkError(f"gk_debug_fetch failed ({rc})")
         return out
+
+    def set_tuning(self, param, value):
+        rc = self.op.gk_set_tuning(self.dev, param, value)
+        if rc != 0:
+            raise GkError(f"gk_set_tuning({param}, {value}) failed ({rc})")
+
+    def get_tuning(self, param):
+        return self.op.gk_get_tuning(self.dev, param)
 
     def device_bytes(self):
         return int(self.op.gk_device_bytes(self.dev))

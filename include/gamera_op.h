@@ -124,6 +124,13 @@ int gk_debug_fetch(gk_solver *s, int which, int field, double *host);
 /* device bytes held by the solver */
 size_t gk_device_bytes(const gk_solver *s);
 
+/* Launch-shape parameters (no counterpart in the reference; they change
+   speed, never results).  GK_TUNE_CHUNK: faces each thread walks in the
+   i and j sweeps (1..4096, default 32). */
+enum { GK_TUNE_CHUNK = 0 };
+int gk_set_tuning(gk_solver *s, int param, int value);
+int gk_get_tuning(const gk_solver *s, int param);
+
 /* Measurement helper (no counterpart in the reference): a plain device copy
    of `bytes` bytes (rounded down to 16-byte units), `reps` times after one
    warm-up; *ms receives the average duration of one copy kernel.  Used to
