@@ -61,36 +61,70 @@ def kernel_alg_bytes(name, n):
     return 0
 
 
-def run_gpu(args):
+def run_form(args, precision):
+    """Time one form of the step on the default workload; returns wall time,
+    per-kernel HIP-event averages, device scalars and the final active state."""
     sys.path.insert(0, PKG)
+    import numpy as np
     import gamera_op as go
 
     n = args.n
-    log(f"bench: host setup {n}^3 (sinusoid + uniform B0 + sphere, seed {args.seed})")
+    form = "FP64" if precision == 0 else "FP32-bulk"
+    log(f"bench: {form}: host setup {n}^3 (sinusoid + uniform B0 + sphere, seed {args.seed})")
     t0 = time.time()
     P = go.Problem(go.SINUSOID_BLAST, args.seed, n, n, n)
     log(f"bench: host setup {time.time() - t0:.1f} s; device open")
-    P.device_open(0)
+    P.device_open(precision)
     if args.chunk:
         P.set_tuning(go.TUNE_CHUNK, args.chunk)
     log(f"bench: device holds {P.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
     P.device_step(args.warmup)
     P.enable_timing(True)
-    log(f"bench: timing {args.steps} steps")
+    log(f"bench: {form}: timing {args.steps} steps")
     t0 = time.perf_counter()
     P.device_step(args.steps)  # returns after the solver's stream has drained
     el = time.perf_counter() - t0
     kt = P.kernel_times()  # average ms per launch over the timed steps
     P.enable_timing(False)
     sc = P.device_scalars()
-    log(f"bench: {el * 1e3 / args.steps:.3f} ms/step; scalars {sc}")
+    log(f"bench: {form}: {el * 1e3 / args.steps:.3f} ms/step; scalars {sc}")
     if sc["nonfinite"]:
         raise RuntimeError("non-finite values in the solution")
-    log("bench: copy calibration")
-    cbytes = 2 * 8 * (n + 9) ** 3 * 6  # read + write of six fields
-    cms = go.copy_calibrate(cbytes // 2, 20)
+    P.device_fetch()
+    a, b, c = slice(4, 4 + n), slice(4, 4 + n), slice(4, 4 + n)
+    state = {"rho": P.gas(0)[a, b, c].copy(), "p": P.gas(4)[a, b, c].copy(),
+             "rhov1": (P.gas(0)[a, b, c] * P.gas(1)[a, b, c]),
+             "Bi": P.gem(0)[4:5 + n, b, c].copy(), "Bj": P.gem(1)[a, 4:5 + n, c].copy(),
+             "Bk": P.gem(2)[a, b, 4:5 + n].copy()}
     P.close()
-    return el, kt, sc, cbytes / (cms * 1e-3) / 1e9
+    return el, kt, sc, state
+
+
+def rel_l1(x, y):
+    import numpy as np
+    return float(np.abs(x - y).sum() / np.abs(y).sum())
+
+
+def run_gpu(args):
+    sys.path.insert(0, PKG)
+    import gamera_op as go
+
+    el, kt, sc, st64 = run_form(args, 0)
+    fp32 = None
+    if not args.no_fp32:
+        el32, kt32, sc32, st32 = run_form(args, 1)
+        cells = args.n ** 3
+        fp32 = {"value": cells * args.steps / el32, "unit": "cell-updates/s",
+                "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": el / el32,
+                "rel_l1_vs_fp64": {k: rel_l1(st32[k], st64[k]) for k in st64},
+                "max_divB": sc32["divB"],
+                "kernel_ms": {k: round(v, 4) for k, v in kt32.items()},
+                "note": "FP64 device path is bitwise equal to the oracle, so this is "
+                        "the error against the FP64 oracle at the same step count"}
+    log("bench: copy calibration")
+    cbytes = 2 * 8 * (args.n + 9) ** 3 * 6  # read + write of six fields
+    cms = go.copy_calibrate(cbytes // 2, 20)
+    return el, kt, sc, cbytes / (cms * 1e-3) / 1e9, fp32
 
 
 def run_cpu(args):
@@ -127,6 +161,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk comparison run")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -139,7 +174,7 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
 
-    el, kt, sc, copy_gbs = run_gpu(args)
+    el, kt, sc, copy_gbs, fp32 = run_gpu(args)
     n = args.n
     cells = n ** 3
     value = cells * args.steps / el
@@ -182,6 +217,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "copy_kernel_GBs": copy_gbs,
         "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
+        "fp32_bulk": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

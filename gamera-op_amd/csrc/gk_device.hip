@@ -41,6 +41,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "gamera_op.h"
 
@@ -65,6 +66,7 @@ struct Geo {
   const double *g[GK_G_COUNT][3];
   const double *cc[3][3];   // cell-centre B coefficients per axis
   const double *recw[3];
+  const float *recwf[3];    // FP32 copies of recw (FP32-bulk form)
   const int *cls[3][2];
   int ncls[3][2];
 };
@@ -86,75 +88,6 @@ struct Scal {
 
 __device__ __forceinline__ double geo3(const Geo &g, int q, int i, int j, int k) {
   return g.g[q][0][i] * g.g[q][1][j] * g.g[q][2][k];
-}
-
-// reconstruction table of sweep axis A at face f for a point with
-// coordinates idx (the two transverse coordinates select the class)
-template <int A>
-__device__ __forceinline__ const double *rtab(const Geo &g, int f, const int idx[3]) {
-  constexpr int B = A == 0 ? 1 : 0;
-  constexpr int C = A == 2 ? 1 : 2;
-  const int cb = g.cls[A][0][idx[B]], ccl = g.cls[A][1][idx[C]];
-  return g.recw[A] + (((long)f * g.ncls[A][0] + cb) * g.ncls[A][1] + ccl) * GK_RECW_STRIDE;
-}
-
-__device__ __forceinline__ double sgn(double x) {
-  return (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0);
-}
-
-// PDMU7 limiter (reference src/solver_func.c:215-253) given the unlimited
-// interpolant and the three limiter cells (i-1, i, i+1 in the upwind frame);
-// the geometry ratio terms (q - e A_i)/(e A_i) and (e A_im1 - q)/(e A_i) are
-// precomputed per face as c1, c2 (the reference forms them first as well).
-__device__ __forceinline__ double pdm_limit(double f_itp, double fm1, double f0, double fp1,
-                                            double c1, double c2) {
-  double maxf = fmax(f0, fp1), minf = fmin(f0, fp1);
-  double f = fmax(minf, fmin(f_itp, maxf));
-  double df0 = f0 - fm1, df1 = fp1 - f0;
-  double s0 = sgn(df0), s1 = sgn(df1);
-  df0 = fabs(df0);
-  double q0 = fabs(s0 + s1);
-  double dfl = f - f0;
-  double df0c = fabs(c1 * f0 + c2 * fm1) / 2.0;
-  df0c = 0.5 * df0c + 0.5 * df0;
-  return f - s1 * fmax(0.0, fabs(dfl) - q0 * df0c);
-}
-
-// Left/right states at a face from 8 stencil cells (f-4 .. f+3): xh are the
-// half-step values that are interpolated, xn the values that limit.
-__device__ __forceinline__ void pdm_lr(const double xh[8], const double xn[8], const double *T,
-                                       double &L, double &R) {
-  double fl = 0.0, fr = 0.0;
-#pragma unroll
-  for (int n = 0; n < 7; n++) {
-    fl += xh[n] * T[n];
-    fr += xh[7 - n] * T[7 + n];
-  }
-  L = pdm_limit(fl, xn[2], xn[3], xn[4], T[14], T[15]);
-  R = pdm_limit(fr, xn[5], xn[4], xn[3], T[16], T[17]);
-}
-
-// Reconstruct a stored field p (history q; self-limited when q == nullptr)
-// at a face along stride sd from offset c.
-__device__ __forceinline__ void recon(const double *__restrict__ p, const double *__restrict__ q,
-                                      long c, long sd, double cc, const double *T, double &L,
-                                      double &R) {
-  double xh[8], xn[8];
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    long o = c + (n - 4) * sd;
-    double x = p[o];
-    xn[n] = x;
-    xh[n] = q ? x + cc * (x - q[o]) : x;
-  }
-  pdm_lr(xh, xn, T, L, R);
-}
-
-__device__ __forceinline__ double entropy_over(double ps, double rho) {
-  // p_S / rho^(gamma-1); p_S is identically zero unless a problem sets it, so
-  // skip the pow in that case (0/x = 0 for finite positive x, as in IEEE).
-  if (ps == 0.0 && rho > 0.0 && rho < INFINITY) return ps;
-  return ps / pow(rho, GAMMA - 1.0);
 }
 
 __device__ __forceinline__ double entropy_times(double s, double rho) {
@@ -227,8 +160,108 @@ __global__ void k_begin(Scal *sc) {
   sc->divb = 0ULL;
 }
 
+__global__ void k_begin_batch(Scal *const *scs, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  Scal *sc = scs[q];
+  double dt = __longlong_as_double((long long)sc->dtmin);
+  sc->dt = dt;
+  sc->time += dt;
+  sc->c = 0.5 * dt / sc->dt0;
+  sc->dt0 = dt;
+  sc->dtmin = 0x7FF0000000000000ULL;
+  sc->divb = 0ULL;
+}
+
 __global__ void k_set_dt0(Scal *sc) {
   sc->dt0 = __longlong_as_double((long long)sc->dtmin);
+}
+
+// Arithmetic of the reconstruction, flux and EMF kernels in real type R.
+// R = double is the reference arithmetic (plain IEEE division and square
+// root, reference operation order).  R = float is the FP32-bulk form: the
+// same formulas with single-precision hardware reciprocal and square root
+// (1 ulp each); state, geometry, CT and the dt reduction stay FP64.
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double qsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double qpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float qpow(float x, float y) { return powf(x, y); }
+
+// reconstruction table of sweep axis A at face f for a point with
+// coordinates idx (the two transverse coordinates select the class)
+template <int A, typename R>
+__device__ __forceinline__ const R *rtab(const Geo &g, int f, const int idx[3]) {
+  constexpr int B = A == 0 ? 1 : 0;
+  constexpr int C = A == 2 ? 1 : 2;
+  const int cb = g.cls[A][0][idx[B]], ccl = g.cls[A][1][idx[C]];
+  const long off = (((long)f * g.ncls[A][0] + cb) * g.ncls[A][1] + ccl) * GK_RECW_STRIDE;
+  if constexpr (sizeof(R) == 8)
+    return (const R *)(g.recw[A] + off);
+  else
+    return (const R *)(g.recwf[A] + off);
+}
+
+template <typename R>
+__device__ __forceinline__ R sgn(R x) {
+  return (x > R(0)) ? R(1) : ((x < R(0)) ? R(-1) : R(0));
+}
+
+// PDMU7 limiter (reference src/solver_func.c:215-253) given the unlimited
+// interpolant and the three limiter cells (i-1, i, i+1 in the upwind frame);
+// the geometry ratio terms (q - e A_i)/(e A_i) and (e A_im1 - q)/(e A_i) are
+// precomputed per face as c1, c2 (the reference forms them first as well).
+template <typename R>
+__device__ __forceinline__ R pdm_limit(R f_itp, R fm1, R f0, R fp1, R c1, R c2) {
+  R maxf = fmax(f0, fp1), minf = fmin(f0, fp1);
+  R f = fmax(minf, fmin(f_itp, maxf));
+  R df0 = f0 - fm1, df1 = fp1 - f0;
+  R s0 = sgn(df0), s1 = sgn(df1);
+  df0 = fabs(df0);
+  R q0 = fabs(s0 + s1);
+  R dfl = f - f0;
+  R df0c = fabs(c1 * f0 + c2 * fm1) / R(2);
+  df0c = R(0.5) * df0c + R(0.5) * df0;
+  return f - s1 * fmax(R(0), fabs(dfl) - q0 * df0c);
+}
+
+// Left/right states at a face from 8 stencil cells (f-4 .. f+3): xh are the
+// half-step values that are interpolated, xn the values that limit.
+template <typename R>
+__device__ __forceinline__ void pdm_lr(const R xh[8], const R xn[8], const R *T, R &L, R &Rr) {
+  R fl = R(0), fr = R(0);
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    fl += xh[n] * T[n];
+    fr += xh[7 - n] * T[7 + n];
+  }
+  L = pdm_limit(fl, xn[2], xn[3], xn[4], T[14], T[15]);
+  Rr = pdm_limit(fr, xn[5], xn[4], xn[3], T[16], T[17]);
+}
+
+// Reconstruct a stored field p (history q; self-limited when q == nullptr)
+// at a face along stride sd from offset c.
+template <typename R, typename S>
+__device__ __forceinline__ void recon(const S *__restrict__ p, const S *__restrict__ q, long c,
+                                      long sd, R cc, const R *T, R &L, R &Rr) {
+  R xh[8], xn[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    long o = c + (n - 4) * sd;
+    R x = (R)p[o];
+    xn[n] = x;
+    xh[n] = q ? x + cc * (x - (R)q[o]) : x;
+  }
+  pdm_lr(xh, xn, T, L, Rr);
+}
+
+template <typename R>
+__device__ __forceinline__ R entropy_over(R ps, R rho) {
+  // p_S / rho^(gamma-1); p_S is identically zero unless a problem sets it, so
+  // skip the pow in that case (0/x = 0 for finite positive x, as in IEEE).
+  if (ps == R(0) && rho > R(0) && rho < R(INFINITY)) return ps;
+  return ps / qpow(rho, R(GAMMA - 1.0));
 }
 
 // Face flux from the reconstructed states: Rusanov hydro flux (reference
@@ -236,65 +269,75 @@ __global__ void k_set_dt0(Scal *sc) {
 // Boris diffusion (src/solver_func.c:1268-1515).  L/R: rho, v1..3, p, p_S;
 // bl/br: cell-centred B (without B0); Bn: face-normal B at the face.
 // Fo: mass, momentum 1..3, energy, entropy, magnetic stress 1..3.
-template <int D>
-__device__ __forceinline__ void face_flux(const double L[6], const double R[6], const double bl[3],
-                                          const double br[3], const double Bn, const B0c &b0c,
-                                          const double CA, double Fo[9]) {
-  const double *b0 = b0c.face[D];
-  const double rl = L[0], rr = R[0], pl = L[4], pr = R[4], psl = L[5], psr = R[5];
-  const double vl[3] = {L[1], L[2], L[3]}, vr[3] = {R[1], R[2], R[3]};
-  double csl = sqrt(GAMMA * pl / rl), csr = sqrt(GAMMA * pr / rr);
-  double cel = sqrt(GAMMA * psl / rl), cer = sqrt(GAMMA * psr / rr);
-  double Btl = (bl[0] + b0[0]) * (bl[0] + b0[0]) + (bl[1] + b0[1]) * (bl[1] + b0[1]) +
-               (bl[2] + b0[2]) * (bl[2] + b0[2]);
-  double Btr = (br[0] + b0[0]) * (br[0] + b0[0]) + (br[1] + b0[1]) * (br[1] + b0[1]) +
-               (br[2] + b0[2]) * (br[2] + b0[2]);
-  double Val = sqrt(Btl / rl), Var = sqrt(Btr / rr);
-  double Vel = Val * CA / (sqrt(Val * Val + CA * CA));
-  double Ver = Var * CA / (sqrt(Var * Var + CA * CA));
-  double vnl = vl[D], vnr = vr[D];
-  double VF = fmax(fabs(vnl) + sqrt(csl * csl + Vel * Vel), fabs(vnr) + sqrt(csr * csr + Ver * Ver));
-  double VFe = fmax(fabs(vnl) + sqrt(cel * cel + Vel * Vel), fabs(vnr) + sqrt(cer * cer + Ver * Ver));
-  double rvl[3], rvr[3];
+template <int D, typename R>
+__device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R bl[3],
+                                          const R br[3], const R Bn, const B0c &b0c,
+                                          const double CAd, R Fo[9]) {
+  const R CA = (R)CAd;
+  const R G = (R)GAMMA;
+  R b0[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) b0[m] = (R)b0c.face[D][m];
+  const R rl = L[0], rr = Rs[0], pl = L[4], pr = Rs[4], psl = L[5], psr = Rs[5];
+  const R vl[3] = {L[1], L[2], L[3]}, vr[3] = {Rs[1], Rs[2], Rs[3]};
+  R csl = qsqrt(qdiv(G * pl, rl)), csr = qsqrt(qdiv(G * pr, rr));
+  R cel = qsqrt(qdiv(G * psl, rl)), cer = qsqrt(qdiv(G * psr, rr));
+  R Btl = (bl[0] + b0[0]) * (bl[0] + b0[0]) + (bl[1] + b0[1]) * (bl[1] + b0[1]) +
+          (bl[2] + b0[2]) * (bl[2] + b0[2]);
+  R Btr = (br[0] + b0[0]) * (br[0] + b0[0]) + (br[1] + b0[1]) * (br[1] + b0[1]) +
+          (br[2] + b0[2]) * (br[2] + b0[2]);
+  R Val = qsqrt(qdiv(Btl, rl)), Var = qsqrt(qdiv(Btr, rr));
+  R Vel = qdiv(Val * CA, qsqrt(Val * Val + CA * CA));
+  R Ver = qdiv(Var * CA, qsqrt(Var * Var + CA * CA));
+  R vnl = vl[D], vnr = vr[D];
+  R VF = fmax(fabs(vnl) + qsqrt(csl * csl + Vel * Vel), fabs(vnr) + qsqrt(csr * csr + Ver * Ver));
+  R VFe = fmax(fabs(vnl) + qsqrt(cel * cel + Vel * Vel), fabs(vnr) + qsqrt(cer * cer + Ver * Ver));
+  R rvl[3], rvr[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     rvl[m] = rl * vl[m];
     rvr[m] = rr * vr[m];
   }
-  double engl = 0.5 * rl * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) + pl / (GAMMA - 1.0);
-  double engr = 0.5 * rr * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) + pr / (GAMMA - 1.0);
-  double Sl = entropy_over(psl, rl), Sr = entropy_over(psr, rr);
-  Fo[0] = 0.5 * (rvl[D] + rvr[D] - VF * (rr - rl));
+  const R h = R(0.5);
+  R engl = h * rl * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) + qdiv(pl, R(GAMMA - 1.0));
+  R engr = h * rr * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) + qdiv(pr, R(GAMMA - 1.0));
+  R Sl = entropy_over(psl, rl), Sr = entropy_over(psr, rr);
+  Fo[0] = h * (rvl[D] + rvr[D] - VF * (rr - rl));
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     if (m == D)
-      Fo[1 + m] = 0.5 * (rvl[m] * vnl + pl + rvr[m] * vnr + pr - VF * (rvr[m] - rvl[m]));
+      Fo[1 + m] = h * (rvl[m] * vnl + pl + rvr[m] * vnr + pr - VF * (rvr[m] - rvl[m]));
     else
-      Fo[1 + m] = 0.5 * (rvl[m] * vnl + rvr[m] * vnr - VF * (rvr[m] - rvl[m]));
+      Fo[1 + m] = h * (rvl[m] * vnl + rvr[m] * vnr - VF * (rvr[m] - rvl[m]));
   }
-  Fo[4] = 0.5 * (engl * vnl + pl * vnl + engr * vnr + pr * vnr - VF * (engr - engl));
-  Fo[5] = 0.5 * (Sl * vnl + Sr * vnr - VFe * (Sr - Sl));
+  Fo[4] = h * (engl * vnl + pl * vnl + engr * vnr + pr * vnr - VF * (engr - engl));
+  Fo[5] = h * (Sl * vnl + Sr * vnr - VFe * (Sr - Sl));
 
-  const double b0n = b0[3];
-  double bsql = (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]) +
-                2 * (bl[0] * b0[0] + bl[1] * b0[1] + bl[2] * b0[2]);
-  double bsqr = (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]) +
-                2 * (br[0] * b0[0] + br[1] * b0[1] + br[2] * b0[2]);
-  double Bsq = (Btl + Btr) / 4 / CA;
+  const R b0n = b0[3];
+  R bsql = (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]) +
+           2 * (bl[0] * b0[0] + bl[1] * b0[1] + bl[2] * b0[2]);
+  R bsqr = (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]) +
+           2 * (br[0] * b0[0] + br[1] * b0[1] + br[2] * b0[2]);
+  R Bsq = qdiv(qdiv(Btl + Btr, R(4)), CA);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
-    double sp, sn;
+    R sp, sn;
     if (m == D) {
-      sp = 0.5 * (0.5 * bsql - (bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = 0.5 * (0.5 * bsqr - (br[m] + b0[m]) * Bn - br[m] * b0n);
+      sp = h * (h * bsql - (bl[m] + b0[m]) * Bn - bl[m] * b0n);
+      sn = h * (h * bsqr - (br[m] + b0[m]) * Bn - br[m] * b0n);
     } else {
-      sp = 0.5 * (-(bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = 0.5 * (-(br[m] + b0[m]) * Bn - br[m] * b0n);
+      sp = h * (-(bl[m] + b0[m]) * Bn - bl[m] * b0n);
+      sn = h * (-(br[m] + b0[m]) * Bn - br[m] * b0n);
     }
-    double ms = sp + sn;
+    R ms = sp + sn;
     Fo[6 + m] = ms - Bsq * (vr[m] - vl[m]);
   }
 }
+
+// work-list entry of a batched launch: argument set and block coordinates
+struct WorkItem {
+  int arg, bx, by, bz;
+};
 
 // ---------------------------------------------------------------------------
 struct SweepArgs {
@@ -304,18 +347,22 @@ struct SweepArgs {
   const double *X, *Xp;  // 6 fields
   const double *M, *Mp;  // 3 cell-B fields at n and n-1
   const double *B;       // face B at n (face-normal component for the stress)
-  double *F;             // 9 fields: 6 hydro + 3 magnetic stress
-  double *V;             // 2 fields: face-averaged velocity components
+  void *F;               // 9 fields of R: 6 hydro + 3 magnetic stress
+  void *V;               // 2 fields of R: face-averaged velocity components
   const Scal *sc;
   int lo[3], hi[3];
   int flo[3], fhi[3];    // range where the full flux is needed
+  int chunk;             // faces per thread in the marching sweeps
 };
 
-template <int D>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
-  const int k = a.lo[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = a.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = a.lo[0] + blockIdx.z;
+// Direct sweep: each thread reads its eight stencil cells (used for the
+// velocity-only strips of the j sweep).
+template <int D, typename R>
+__device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
+                                        const int bz) {
+  const int k = a.lo[2] + bx * blockDim.x + threadIdx.x;
+  const int j = a.lo[1] + by * 4 + threadIdx.y;
+  const int i = a.lo[0] + bz;
   if (k > a.hi[2] || j > a.hi[1]) return;
   const Grid &g = a.g;
   const long c = i * g.st[0] + j * g.st[1] + k;
@@ -324,30 +371,47 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
   const int f = idx[D];
   const bool full = i >= a.flo[0] && i <= a.fhi[0] && j >= a.flo[1] && j <= a.fhi[1] &&
                     k >= a.flo[2] && k <= a.fhi[2];
-  const double cc = a.sc->c;
-  const double *T = rtab<D>(a.geo, f, idx);
+  const R cc = (R)a.sc->c;
+  const R *T = rtab<D, R>(a.geo, f, idx);
+  R *F = (R *)a.F, *V = (R *)a.V;
 
   // velocity components that feed the edge fields: D=0 (v1,v2), 1 (v2,v3), 2 (v3,v1)
   constexpr int va = D == 0 ? 1 : (D == 1 ? 2 : 3);
   constexpr int vb = D == 0 ? 2 : (D == 1 ? 3 : 1);
-  double L[6], R[6];
+  R L[6], Rs[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) {
     if (!full && m != va && m != vb) continue;
-    recon(a.X + m * g.fs, a.Xp + m * g.fs, c, sd, cc, T, L[m], R[m]);
+    recon(a.X + m * g.fs, a.Xp + m * g.fs, c, sd, cc, T, L[m], Rs[m]);
   }
-  a.V[c] = 0.5 * (L[va] + R[va]);
-  a.V[g.fs + c] = 0.5 * (L[vb] + R[vb]);
+  V[c] = R(0.5) * (L[va] + Rs[va]);
+  V[g.fs + c] = R(0.5) * (L[vb] + Rs[vb]);
   if (!full) return;
 
-  double bl[3], br[3];
+  R bl[3], br[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) recon(a.M + m * g.fs, a.Mp + m * g.fs, c, sd, cc, T, bl[m], br[m]);
 
-  double Fo[9];
-  face_flux<D>(L, R, bl, br, a.B[D * g.fs + c], a.b0, g.CA, Fo);
+  R Fo[9];
+  face_flux<D, R>(L, Rs, bl, br, (R)a.B[D * g.fs + c], a.b0, g.CA, Fo);
 #pragma unroll
-  for (int m = 0; m < 9; m++) a.F[m * g.fs + c] = Fo[m];
+  for (int m = 0; m < 9; m++) F[m * g.fs + c] = Fo[m];
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
+  k_sweep_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_sweep_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
 }
 
 // Marching sweep along D = 0 (i) or 1 (j).  Lanes run along k (coalesced),
@@ -357,25 +421,25 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
 // limiter need: each face loads one new cell instead of eight.  Rows outside
 // [flo, fhi] of RA only form the two face-averaged velocities.  The per-face
 // arithmetic is that of k_sweep.
-template <int NF>
+template <int NF, typename R>
 struct Window {
-  double h[NF][8];  // half-step values at cells f-4 .. f+3
-  double x[NF][6];  // current values at cells f-2 .. f+3
+  R h[NF][8];  // half-step values at cells f-4 .. f+3
+  R x[NF][6];  // current values at cells f-2 .. f+3
 };
 
-template <int D, int NF>
-__device__ __forceinline__ void march_load(Window<NF> &w, const double *const P[NF],
-                                           const double *const Q[NF], long o, double cc, int slot) {
+template <int NF, typename R>
+__device__ __forceinline__ void march_load(Window<NF, R> &w, const double *const P[NF],
+                                           const double *const Q[NF], long o, R cc, int slot) {
 #pragma unroll
   for (int m = 0; m < NF; m++) {
-    const double x = P[m][o];
-    w.h[m][slot] = x + cc * (x - Q[m][o]);
+    const R x = (R)P[m][o];
+    w.h[m][slot] = x + cc * (x - (R)Q[m][o]);
     if (slot >= 2) w.x[m][slot - 2] = x;
   }
 }
 
-template <int NF>
-__device__ __forceinline__ void march_shift(Window<NF> &w) {
+template <int NF, typename R>
+__device__ __forceinline__ void march_shift(Window<NF, R> &w) {
 #pragma unroll
   for (int m = 0; m < NF; m++) {
 #pragma unroll
@@ -385,10 +449,9 @@ __device__ __forceinline__ void march_shift(Window<NF> &w) {
   }
 }
 
-template <int NF>
-__device__ __forceinline__ void march_lr(const Window<NF> &w, int m, const double *T, double &L,
-                                         double &R) {
-  double fl = 0.0, fr = 0.0;
+template <int NF, typename R>
+__device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R *T, R &L, R &Rr) {
+  R fl = R(0), fr = R(0);
 #pragma unroll
   for (int n = 0; n < 7; n++) {
     fl += w.h[m][n] * T[n];
@@ -396,17 +459,19 @@ __device__ __forceinline__ void march_lr(const Window<NF> &w, int m, const doubl
   }
   // limiter cells: x[0] = f-2, x[1] = f-1, x[2] = f, x[3] = f+1
   L = pdm_limit(fl, w.x[m][0], w.x[m][1], w.x[m][2], T[14], T[15]);
-  R = pdm_limit(fr, w.x[m][3], w.x[m][2], w.x[m][1], T[16], T[17]);
+  Rr = pdm_limit(fr, w.x[m][3], w.x[m][2], w.x[m][1], T[16], T[17]);
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
+template <int D, typename R>
+__device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, const int by,
+                                        const int bz) {
+  const int chunk = a.chunk;
   constexpr int RA = D == 0 ? 1 : 0;  // row axis
   const Grid &g = a.g;
-  const int k = a.lo[2] + blockIdx.x * 64 + threadIdx.x;
-  const int r = a.lo[RA] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = a.lo[2] + bx * 64 + threadIdx.x;
+  const int r = a.lo[RA] + by * 4 + threadIdx.y;
   if (k > a.hi[2] || r > a.hi[RA]) return;
-  const int f0 = a.lo[D] + blockIdx.z * chunk;
+  const int f0 = a.lo[D] + bz * chunk;
   const int f1 = min(f0 + chunk, a.hi[D] + 1);
   int idx[3];
   idx[2] = k;
@@ -414,12 +479,13 @@ __global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
   idx[D] = f0;
   const long sd = g.st[D];
   long c = idx[0] * g.st[0] + idx[1] * g.st[1] + k;
-  const double cc = a.sc->c;
-  const double *T = rtab<D>(a.geo, f0, idx);
+  const R cc = (R)a.sc->c;
+  const R *T = rtab<D, R>(a.geo, f0, idx);
   const long tstep = (long)a.geo.ncls[D][0] * a.geo.ncls[D][1] * GK_RECW_STRIDE;
   constexpr int va = D == 0 ? 1 : 2;
   constexpr int vb = D == 0 ? 2 : 3;
   const long fs = g.fs;
+  R *F = (R *)a.F, *V = (R *)a.V;
   if (r >= a.flo[RA] && r <= a.fhi[RA]) {
     const double *P[9], *Q[9];
 #pragma unroll
@@ -432,22 +498,22 @@ __global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
       P[6 + m] = a.M + m * fs;
       Q[6 + m] = a.Mp + m * fs;
     }
-    Window<9> w;
+    Window<9, R> w;
 #pragma unroll
-    for (int p = 0; p < 7; p++) march_load<D, 9>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int p = 0; p < 7; p++) march_load<9, R>(w, P, Q, c + (p - 4) * sd, cc, p);
     for (int f = f0; f < f1; f++) {
-      march_load<D, 9>(w, P, Q, c + 3 * sd, cc, 7);
-      double L[6], R[6], bl[3], br[3];
+      march_load<9, R>(w, P, Q, c + 3 * sd, cc, 7);
+      R L[6], Rs[6], bl[3], br[3];
 #pragma unroll
-      for (int m = 0; m < 6; m++) march_lr(w, m, T, L[m], R[m]);
+      for (int m = 0; m < 6; m++) march_lr(w, m, T, L[m], Rs[m]);
 #pragma unroll
       for (int m = 0; m < 3; m++) march_lr(w, 6 + m, T, bl[m], br[m]);
-      a.V[c] = 0.5 * (L[va] + R[va]);
-      a.V[fs + c] = 0.5 * (L[vb] + R[vb]);
-      double Fo[9];
-      face_flux<D>(L, R, bl, br, a.B[D * fs + c], a.b0, g.CA, Fo);
+      V[c] = R(0.5) * (L[va] + Rs[va]);
+      V[fs + c] = R(0.5) * (L[vb] + Rs[vb]);
+      R Fo[9];
+      face_flux<D, R>(L, Rs, bl, br, (R)a.B[D * fs + c], a.b0, g.CA, Fo);
 #pragma unroll
-      for (int m = 0; m < 9; m++) a.F[m * fs + c] = Fo[m];
+      for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
       march_shift(w);
       c += sd;
       T += tstep;
@@ -455,20 +521,36 @@ __global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
   } else {
     const double *P[2] = {a.X + va * fs, a.X + vb * fs};
     const double *Q[2] = {a.Xp + va * fs, a.Xp + vb * fs};
-    Window<2> w;
+    Window<2, R> w;
 #pragma unroll
-    for (int p = 0; p < 7; p++) march_load<D, 2>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int p = 0; p < 7; p++) march_load<2, R>(w, P, Q, c + (p - 4) * sd, cc, p);
     for (int f = f0; f < f1; f++) {
-      march_load<D, 2>(w, P, Q, c + 3 * sd, cc, 7);
-      double l, rr;
+      march_load<2, R>(w, P, Q, c + 3 * sd, cc, 7);
+      R l, rr;
       march_lr(w, 0, T, l, rr);
-      a.V[c] = 0.5 * (l + rr);
+      V[c] = R(0.5) * (l + rr);
       march_lr(w, 1, T, l, rr);
-      a.V[fs + c] = 0.5 * (l + rr);
+      V[fs + c] = R(0.5) * (l + rr);
       march_shift(w);
       c += sd;
       T += tstep;
     }
+  }
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_march(SweepArgs a) {
+  k_march_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_march_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
   }
 }
 
@@ -480,71 +562,134 @@ __global__ __launch_bounds__(256) void k_march(SweepArgs a, int chunk) {
 constexpr int KROWS = 4;
 constexpr int KSPAN = 72;
 
-__global__ __launch_bounds__(256) void k_sweepk(SweepArgs a) {
-  __shared__ double sh[9][KROWS][KSPAN];
-  __shared__ double sx[9][KROWS][KSPAN];
+template <typename R>
+__device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, const int by,
+                                         const int bz) {
+  __shared__ R sh[9][KROWS][KSPAN];
+  __shared__ R sx[9][KROWS][KSPAN];
   const Grid &g = a.g;
-  const int k0 = a.lo[2] + blockIdx.x * 64;
-  const int j0 = a.lo[1] + blockIdx.y * KROWS;
-  const int i = a.lo[0] + blockIdx.z;
+  const int k0 = a.lo[2] + bx * 64;
+  const int j0 = a.lo[1] + by * KROWS;
+  const int i = a.lo[0] + bz;
   const long fs = g.fs;
   const bool full = i >= a.flo[0] && i <= a.fhi[0];
-  const double cc = a.sc->c;
-  const int tid = threadIdx.x + 64 * threadIdx.y;
-  // stage: points k0-4 .. k0+67 of rows j0 .. j0+KROWS-1 (clamped to the box)
-  const int nf = full ? 9 : 2;
-  for (int q = tid; q < nf * KROWS * KSPAN; q += 256) {
-    const int m = q / (KROWS * KSPAN);
-    const int rr = (q / KSPAN) % KROWS;
-    const int kk = q % KSPAN;
-    const int field = full ? m : (m == 0 ? 3 : 1);
-    const int slot = full ? m : (m == 0 ? 3 : 1);
-    const int j = min(j0 + rr, g.N[1] - 1);
-    const int kx = min(k0 - 4 + kk, g.N[2] - 1);
-    const long o = i * g.st[0] + j * g.st[1] + kx;
-    const double *P = field < 6 ? a.X + field * fs : a.M + (field - 6) * fs;
-    const double *Q = field < 6 ? a.Xp + field * fs : a.Mp + (field - 6) * fs;
-    const double x = P[o];
-    sh[slot][rr][kk] = x + cc * (x - Q[o]);
-    sx[slot][rr][kk] = x;
+  const R cc = (R)a.sc->c;
+  R *F = (R *)a.F, *V = (R *)a.V;
+  // stage: points k0-4 .. k0+67 of rows j0 .. j0+KROWS-1 (clamped to the box);
+  // lane tx loads point tx and, for tx < 8, point 64 + tx of its row
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const long rowb = i * g.st[0] + min(j0 + ty, g.N[1] - 1) * g.st[1];
+  const long oa = rowb + min(k0 - 4 + tx, g.N[2] - 1);
+  const long ob = rowb + min(k0 + 60 + tx, g.N[2] - 1);
+  auto stage = [&](int m, const double *P, const double *Q) {
+    const R xa = (R)P[oa], pa = (R)Q[oa];
+    sh[m][ty][tx] = xa + cc * (xa - pa);
+    sx[m][ty][tx] = xa;
+    if (tx < KSPAN - 64) {
+      const R xb = (R)P[ob], pb = (R)Q[ob];
+      sh[m][ty][64 + tx] = xb + cc * (xb - pb);
+      sx[m][ty][64 + tx] = xb;
+    }
+  };
+  if (full) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) stage(m, a.X + m * fs, a.Xp + m * fs);
+#pragma unroll
+    for (int m = 0; m < 3; m++) stage(6 + m, a.M + m * fs, a.Mp + m * fs);
+  } else {
+    stage(3, a.X + 3 * fs, a.Xp + 3 * fs);
+    stage(1, a.X + 1 * fs, a.Xp + 1 * fs);
   }
   __syncthreads();
   const int k = k0 + threadIdx.x;
   const int j = j0 + threadIdx.y;
   if (k > a.hi[2] || j > a.hi[1]) return;
   int idx[3] = {i, j, k};
-  const double *T = rtab<2>(a.geo, k, idx);
+  const R *T = rtab<2, R>(a.geo, k, idx);
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int ry = threadIdx.y, b = threadIdx.x;  // stencil cells b .. b+7 in LDS
-  auto lr = [&](int m, double &L, double &R) {
-    double fl = 0.0, fr = 0.0;
+  auto lr = [&](int m, R &L, R &Rr) {
+    R fl = R(0), fr = R(0);
 #pragma unroll
     for (int n = 0; n < 7; n++) {
       fl += sh[m][ry][b + n] * T[n];
       fr += sh[m][ry][b + 7 - n] * T[7 + n];
     }
     L = pdm_limit(fl, sx[m][ry][b + 2], sx[m][ry][b + 3], sx[m][ry][b + 4], T[14], T[15]);
-    R = pdm_limit(fr, sx[m][ry][b + 5], sx[m][ry][b + 4], sx[m][ry][b + 3], T[16], T[17]);
+    Rr = pdm_limit(fr, sx[m][ry][b + 5], sx[m][ry][b + 4], sx[m][ry][b + 3], T[16], T[17]);
   };
   if (!full) {
-    double l, r;
+    R l, r;
     lr(3, l, r);
-    a.V[c] = 0.5 * (l + r);
+    V[c] = R(0.5) * (l + r);
     lr(1, l, r);
-    a.V[fs + c] = 0.5 * (l + r);
+    V[fs + c] = R(0.5) * (l + r);
     return;
   }
-  double L[6], R[6], bl[3], br[3];
+  R L[6], Rs[6], bl[3], br[3];
 #pragma unroll
-  for (int m = 0; m < 6; m++) lr(m, L[m], R[m]);
+  for (int m = 0; m < 6; m++) lr(m, L[m], Rs[m]);
 #pragma unroll
   for (int m = 0; m < 3; m++) lr(6 + m, bl[m], br[m]);
-  a.V[c] = 0.5 * (L[3] + R[3]);
-  a.V[fs + c] = 0.5 * (L[1] + R[1]);
-  double Fo[9];
-  face_flux<2>(L, R, bl, br, a.B[2 * fs + c], a.b0, g.CA, Fo);
+  V[c] = R(0.5) * (L[3] + Rs[3]);
+  V[fs + c] = R(0.5) * (L[1] + Rs[1]);
+  R Fo[9];
+  face_flux<2, R>(L, Rs, bl, br, (R)a.B[2 * fs + c], a.b0, g.CA, Fo);
 #pragma unroll
-  for (int m = 0; m < 9; m++) a.F[m * fs + c] = Fo[m];
+  for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sweepk(SweepArgs a) {
+  k_sweepk_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <typename R>
+__global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_sweepk_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
+}
+
+// Edge EMF from the edge velocities (va, vb), the two reconstructed face-B
+// components (b1 = field F1 along R1, b2 = field F2 along R2) and the sum of
+// the four half-step densities around the edge (reference
+// src/solver_func.c:779-932; E_i: va = v2, vb = v3, b1 = B_j, b2 = B_k;
+// E_j: va = v3, vb = v1, b1 = B_k, b2 = B_i; E_k: va = v1, vb = v2,
+// b1 = B_i, b2 = B_j).
+template <int D, typename R>
+__device__ __forceinline__ R emf_formula(R va, R vb, R b1l, R b1r, R b2l, R b2r, R rsum,
+                                         const double *e0d, double CAd) {
+  const R CA = (R)CAd, h = R(0.5);
+  const R e0[3] = {(R)e0d[0], (R)e0d[1], (R)e0d[2]};
+  R e, dv;
+  if (D == 0) {
+    R bja = h * (b1l + b1r) + e0[1];
+    R bka = h * (b2l + b2r) + e0[2];
+    e = -(va * bka - vb * bja);
+    dv = qsqrt(qdiv(bja * bja + bka * bka, R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(va * va + vb * vb);
+    e += h * dv * (b1l + b2r - b1r - b2l);
+  } else if (D == 1) {
+    R bka = h * (b1l + b1r) + e0[2];
+    R bia = h * (b2l + b2r) + e0[0];
+    e = -(va * bia - vb * bka);
+    dv = qsqrt(qdiv(bka * bka + bia * bia, R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(vb * vb + va * va);
+    e += h * dv * (b1l + b2r - b1r - b2l);
+  } else {
+    R bia = h * (b1l + b1r) + e0[0];
+    R bja = h * (b2l + b2r) + e0[1];
+    e = -(va * bja - vb * bia);
+    dv = qsqrt(qdiv(bia * bia + bja * bja, R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(vb * vb + va * va);
+    e += h * dv * (b1l + b2r - b1r - b2l);
+  }
+  return e;
 }
 
 // ---------------------------------------------------------------------------
@@ -554,78 +699,155 @@ struct EmfArgs {
   B0c b0;
   const double *X, *Xp;
   const double *B, *Bp;
-  const double *V;  // 2 fields of face-averaged velocity (from the feeding sweep)
-  double *E;
+  const void *V;  // 2 fields of R: face-averaged velocity (from the feeding sweep)
+  void *E;        // 1 field of R
   const Scal *sc;
   int lo[3], hi[3];
+  int chunk;      // edges per thread along the march axis
 };
 
-// Edge EMF (reference src/solver_func.c:779-932).  D is the edge direction.
-template <int D>
-__global__ __launch_bounds__(256) void k_emf(EmfArgs a) {
-  const int k = a.lo[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = a.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = a.lo[0] + blockIdx.z;
-  if (k > a.hi[2] || j > a.hi[1]) return;
-  const Grid &g = a.g;
-  const long c = i * g.st[0] + j * g.st[1] + k;
-  const int idx[3] = {i, j, k};
-  const double cc = a.sc->c;
-  const double CA = g.CA;
-  // velocity sweep-2 axis: E_i along k, E_j along i, E_k along j
-  constexpr int A2 = D == 0 ? 2 : (D == 1 ? 0 : 1);
-  const double *T2 = rtab<A2>(a.geo, idx[A2], idx);
-  double l, r;
-  recon(a.V, nullptr, c, g.st[A2], cc, T2, l, r);
-  const double va = 0.5 * (l + r);
-  recon(a.V + g.fs, nullptr, c, g.st[A2], cc, T2, l, r);
-  const double vb = 0.5 * (l + r);
-  // face-B reconstructions to the edge: E_i: B_j along k, B_k along j;
-  // E_j: B_k along i, B_i along k; E_k: B_i along j, B_j along i
-  constexpr int F1 = D == 0 ? 1 : (D == 1 ? 2 : 0);  // field
-  constexpr int R1 = D == 0 ? 2 : (D == 1 ? 0 : 1);  // recon axis
+// Marching EMF along the axis MA that two of the three reconstructions share
+// (E_i: j, with B_k windowed and the k-reconstructions of v and B_j read
+// directly; E_j: i, v and B_k windowed; E_k: j, v and B_i windowed).  Lanes
+// run along k; the four half-step densities around the edge are carried
+// from one march position to the next.  E_i on edges (i, j-1/2, k-1/2) etc.,
+// edge ranges [s, e+1] across and [s, e] along the edge.
+template <typename R>
+struct Window1 {  // self-limited field: half-step = current value
+  R h[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void lr_self(const Window1<R> &w, const R *T, R &L, R &Rr) {
+  R fl = R(0), fr = R(0);
+#pragma unroll
+  for (int n = 0; n < 7; n++) {
+    fl += w.h[n] * T[n];
+    fr += w.h[7 - n] * T[7 + n];
+  }
+  L = pdm_limit(fl, w.h[2], w.h[3], w.h[4], T[14], T[15]);
+  Rr = pdm_limit(fr, w.h[5], w.h[4], w.h[3], T[16], T[17]);
+}
+
+template <typename R>
+__device__ __forceinline__ void shift1(Window1<R> &w) {
+#pragma unroll
+  for (int n = 0; n < 7; n++) w.h[n] = w.h[n + 1];
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
+                                            const int bz) {
+  const int chunk = a.chunk;
+  constexpr int MA = D == 1 ? 0 : 1;  // march axis
+  constexpr int RA = D == 1 ? 1 : 0;  // row axis
+  constexpr int A2 = D == 0 ? 2 : (D == 1 ? 0 : 1);  // velocity (and F1) axis
+  constexpr int F1 = D == 0 ? 1 : (D == 1 ? 2 : 0);
   constexpr int F2 = D == 0 ? 2 : (D == 1 ? 0 : 1);
   constexpr int R2 = D == 0 ? 1 : (D == 1 ? 2 : 0);
-  double b1l, b1r, b2l, b2r;
-  recon(a.B + F1 * g.fs, a.Bp + F1 * g.fs, c, g.st[R1], cc, rtab<R1>(a.geo, idx[R1], idx), b1l,
-        b1r);
-  recon(a.B + F2 * g.fs, a.Bp + F2 * g.fs, c, g.st[R2], cc, rtab<R2>(a.geo, idx[R2], idx), b2l,
-        b2r);
-  // half-step density on the four cells around the edge
-  constexpr int P = D == 0 ? 1 : 0;
+  constexpr int P = D == 0 ? 1 : 0;  // density cells: c - st[P] - st[Q] ... c
   constexpr int Q = D == 2 ? 1 : 2;
-  const long sp = g.st[P], sq = g.st[Q];
+  constexpr int O = P == MA ? Q : P;  // density axis other than the march axis
+  constexpr bool VW = A2 == MA;       // v and F1 windowed
+  const Grid &g = a.g;
+  const int k = a.lo[2] + bx * 64 + threadIdx.x;
+  const int r = a.lo[RA] + by * 4 + threadIdx.y;
+  if (k > a.hi[2] || r > a.hi[RA]) return;
+  const int f0 = a.lo[MA] + bz * chunk;
+  const int f1 = min(f0 + chunk, a.hi[MA] + 1);
+  int idx[3];
+  idx[2] = k;
+  idx[RA] = r;
+  idx[MA] = f0;
+  const long sm = g.st[MA], so = g.st[O];
+  const long fs = g.fs;
+  long c = idx[0] * g.st[0] + idx[1] * g.st[1] + k;
+  const R cc = (R)a.sc->c;
+  const R *V0 = (const R *)a.V, *V1 = (const R *)a.V + fs;
+  R *E = (R *)a.E;
+  const double *B1 = a.B + F1 * fs, *B1p = a.Bp + F1 * fs;
+  const double *B2 = a.B + F2 * fs, *B2p = a.Bp + F2 * fs;
   const double *X = a.X, *Xp = a.Xp;
   auto rh = [&](long o) {
-    double x = X[o];
-    return x + cc * (x - Xp[o]);
+    R x = (R)X[o];
+    return x + cc * (x - (R)Xp[o]);
   };
-  const double rsum = rh(c - sp - sq) + rh(c - sp) + rh(c - sq) + rh(c);
-  const double *e0 = a.b0.edge[D];
-  double e, dv;
-  if (D == 0) {  // va = v2, vb = v3; bj = field 1 along k, bk = field 2 along j
-    double bja = 0.5 * (b1l + b1r) + e0[1];
-    double bka = 0.5 * (b2l + b2r) + e0[2];
-    e = -(va * bka - vb * bja);
-    dv = sqrt((bja * bja + bka * bka) / (0.25 * rsum));
-    dv = dv * CA / sqrt(dv * dv + CA * CA) + sqrt(va * va + vb * vb);
-    e += 0.5 * dv * (b1l + b2r - b1r - b2l);
-  } else if (D == 1) {  // va = v3, vb = v1; bk = field 2 along i, bi = field 0 along k
-    double bka = 0.5 * (b1l + b1r) + e0[2];
-    double bia = 0.5 * (b2l + b2r) + e0[0];
-    e = -(va * bia - vb * bka);
-    dv = sqrt((bka * bka + bia * bia) / (0.25 * rsum));
-    dv = dv * CA / sqrt(dv * dv + CA * CA) + sqrt(vb * vb + va * va);
-    e += 0.5 * dv * (b1l + b2r - b1r - b2l);
-  } else {  // va = v1, vb = v2; bi = field 0 along j, bj = field 1 along i
-    double bia = 0.5 * (b1l + b1r) + e0[0];
-    double bja = 0.5 * (b2l + b2r) + e0[1];
-    e = -(va * bja - vb * bia);
-    dv = sqrt((bia * bia + bja * bja) / (0.25 * rsum));
-    dv = dv * CA / sqrt(dv * dv + CA * CA) + sqrt(vb * vb + va * va);
-    e += 0.5 * dv * (b1l + b2r - b1r - b2l);
+  const R *Tm = rtab<MA, R>(a.geo, f0, idx);  // windowed reconstructions
+  const long tstep = (long)a.geo.ncls[MA][0] * a.geo.ncls[MA][1] * GK_RECW_STRIDE;
+  Window1<R> wv0, wv1;
+  Window<1, R> wb;
+  const double *Bw = VW ? B1 : B2, *Bwp = VW ? B1p : B2p;
+  const double *Pw[1] = {Bw}, *Qw[1] = {Bwp};
+#pragma unroll
+  for (int p = 0; p < 7; p++) {
+    const long o = c + (p - 4) * sm;
+    if (VW) {
+      wv0.h[p] = V0[o];
+      wv1.h[p] = V1[o];
+    }
+    march_load<1, R>(wb, Pw, Qw, o, cc, p);
   }
-  a.E[c] = e;
+  // densities at march position f0 - 1 (offsets -1 and 0 along O)
+  R u0 = rh(c - sm - so), u1 = rh(c - sm);
+  for (int f = f0; f < f1; f++) {
+    const long o = c + 3 * sm;
+    if (VW) {
+      wv0.h[7] = V0[o];
+      wv1.h[7] = V1[o];
+    }
+    march_load<1, R>(wb, Pw, Qw, o, cc, 7);
+    idx[MA] = f;
+    R va, vb, b1l, b1r, b2l, b2r, l, rr;
+    if (VW) {
+      lr_self(wv0, Tm, l, rr);
+      va = R(0.5) * (l + rr);
+      lr_self(wv1, Tm, l, rr);
+      vb = R(0.5) * (l + rr);
+      march_lr(wb, 0, Tm, b1l, b1r);
+      recon(B2, B2p, c, g.st[R2], cc, rtab<R2, R>(a.geo, idx[R2], idx), b2l, b2r);
+    } else {
+      const R *T2 = rtab<A2, R>(a.geo, idx[A2], idx);
+      recon(V0, (const R *)nullptr, c, g.st[A2], cc, T2, l, rr);
+      va = R(0.5) * (l + rr);
+      recon(V1, (const R *)nullptr, c, g.st[A2], cc, T2, l, rr);
+      vb = R(0.5) * (l + rr);
+      recon(B1, B1p, c, g.st[A2], cc, T2, b1l, b1r);
+      march_lr(wb, 0, Tm, b2l, b2r);
+    }
+    const R w0 = rh(c - so), w1 = rh(c);
+    // reference order: rho(c-sP-sQ) + rho(c-sP) + rho(c-sQ) + rho(c)
+    R rsum;
+    if (Q == MA)
+      rsum = u0 + w0 + u1 + w1;
+    else
+      rsum = u0 + u1 + w0 + w1;
+    E[c] = emf_formula<D, R>(va, vb, b1l, b1r, b2l, b2r, rsum, a.b0.edge[D], g.CA);
+    u0 = w0;
+    u1 = w1;
+    if (VW) {
+      shift1(wv0);
+      shift1(wv1);
+    }
+    march_shift(wb);
+    c += sm;
+    Tm += tstep;
+  }
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_emf_march(EmfArgs a) {
+  k_emf_march_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_emf_march_batch(const EmfArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_emf_march_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -634,37 +856,57 @@ struct CtArgs {
   Geo geo;
   const double *B;   // B^n
   double *Bn;        // B^{n+1}
-  const double *E;   // 3 edge fields
+  const void *E;     // 3 edge fields of R
   const Scal *sc;
 };
 
-// Constrained-transport face update (reference src/solver_solve.c:262-306)
-__global__ __launch_bounds__(256) void k_ct(CtArgs a) {
+// Constrained-transport face update (reference src/solver_solve.c:262-306),
+// in FP64 for both forms (EMFs of type R are widened first), so the discrete
+// div B stays at FP64 roundoff.
+template <typename R>
+__device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const int by,
+                                     const int bz) {
   const Grid &g = a.g;
-  const int k = g.s[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.s[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = g.s[0] + blockIdx.z;
+  const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
+  const int j = g.s[1] + by * 4 + threadIdx.y;
+  const int i = g.s[0] + bz;
   if (k > g.e[2] + 1 || j > g.e[1] + 1) return;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const long SI = g.st[0], SJ = g.st[1];
   const double dt = a.sc->dt;
-  const double *Ei = a.E, *Ej = a.E + g.fs, *Ek = a.E + 2 * g.fs;
+  const R *Ei = (const R *)a.E, *Ej = Ei + g.fs, *Ek = Ei + 2 * g.fs;
   const Geo &G = a.geo;
   const bool ai = i <= g.e[0], aj = j <= g.e[1], ak = k <= g.e[2];
   if (aj && ak) {  // B_i on faces i = is..ie+1
-    double d = (Ek[c + SJ] * geo3(G, GK_G_EDGE_K, i, j + 1, k) - Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k)) -
-               (Ej[c + 1] * geo3(G, GK_G_EDGE_J, i, j, k + 1) - Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k));
+    double d = ((double)Ek[c + SJ] * geo3(G, GK_G_EDGE_K, i, j + 1, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k)) -
+               ((double)Ej[c + 1] * geo3(G, GK_G_EDGE_J, i, j, k + 1) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k));
     a.Bn[c] = a.B[c] - dt * d / geo3(G, GK_G_FACE_I, i, j, k);
   }
   if (ai && ak) {  // B_j on faces j = js..je+1
-    double d = (Ei[c + 1] * geo3(G, GK_G_EDGE_I, i, j, k + 1) - Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k)) -
-               (Ek[c + SI] * geo3(G, GK_G_EDGE_K, i + 1, j, k) - Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k));
+    double d = ((double)Ei[c + 1] * geo3(G, GK_G_EDGE_I, i, j, k + 1) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k)) -
+               ((double)Ek[c + SI] * geo3(G, GK_G_EDGE_K, i + 1, j, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k));
     a.Bn[g.fs + c] = a.B[g.fs + c] - dt * d / geo3(G, GK_G_FACE_J, i, j, k);
   }
   if (ai && aj) {  // B_k on faces k = ks..ke+1
-    double d = (Ej[c + SI] * geo3(G, GK_G_EDGE_J, i + 1, j, k) - Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k)) -
-               (Ei[c + SJ] * geo3(G, GK_G_EDGE_I, i, j + 1, k) - Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k));
+    double d = ((double)Ej[c + SI] * geo3(G, GK_G_EDGE_J, i + 1, j, k) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k)) -
+               ((double)Ei[c + SJ] * geo3(G, GK_G_EDGE_I, i, j + 1, k) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k));
     a.Bn[2 * g.fs + c] = a.B[2 * g.fs + c] - dt * d / geo3(G, GK_G_FACE_K, i, j, k);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_ct(CtArgs a) {
+  k_ct_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <typename R>
+__global__ __launch_bounds__(256) void k_ct_batch(const CtArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_ct_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
   }
 }
 
@@ -678,7 +920,7 @@ struct CellArgs {
   double *Xn;              // X^{n+1} (aliases Xp: each cell reads before it writes)
   const double *M, *Mp;    // cell B at n, n-1
   const double *Bn;        // face B at n+1
-  const double *F[3];      // 9 fields per direction
+  const void *F[3];        // 9 fields of R per direction
   Scal *sc;
 };
 
@@ -686,12 +928,14 @@ struct CellArgs {
 // src/solver_func.c:2117-2135; spherical sources
 // src/spherical/spherical_general.c:605-688) plus the next step's CFL dt and
 // |div B| (src/solver_func.c:255-343).
-__global__ __launch_bounds__(256) void k_cell(CellArgs a) {
+template <typename R>
+__device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, const int by,
+                                       const int bz) {
   const Grid &g = a.g;
   const Geo &G = a.geo;
-  const int k = g.s[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.s[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = g.s[0] + blockIdx.z;
+  const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
+  const int j = g.s[1] + by * 4 + threadIdx.y;
+  const int i = g.s[0] + bz;
   const bool valid = k <= g.e[2] && j <= g.e[1];
   double dtc = INFINITY, divb = 0.0;
   bool bad = false;
@@ -714,11 +958,11 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a) {
     for (int m = 0; m < 9; m++) dU[m] = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const double *Fd = a.F[d];
+      const R *Fd = (const R *)a.F[d];
       const long sd = g.st[d];
 #pragma unroll
       for (int m = 0; m < 9; m++) {
-        double v = (Fd[m * g.fs + c + sd] * A1[d] - Fd[m * g.fs + c] * A0[d]) / V;
+        double v = ((double)Fd[m * g.fs + c + sd] * A1[d] - (double)Fd[m * g.fs + c] * A0[d]) / V;
         dU[m] = d == 0 ? v : dU[m] + v;
       }
     }
@@ -744,7 +988,7 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a) {
       double ph = x[4] + cc * (x[4] - xp[4]);
       double M22 = ph + rh * v2h * v2h;
       double M33 = ph + rh * v3h * v3h;
-      const double *Fi = a.F[0], *Fj = a.F[1];
+      const R *Fi = (const R *)a.F[0], *Fj = (const R *)a.F[1];
       const long SI = g.st[0], SJ = g.st[1];
       const double dx1 = G.g[GK_G_EDGE_I][0][i];  // dx1
       const double x1c = G.g[GK_G_DXJ][0][i];     // x1c (cell-centre radius)
@@ -844,6 +1088,22 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a) {
   if (bad) atomicOr(&a.sc->nanflag, 1u);
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_cell(CellArgs a) {
+  k_cell_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+template <typename R>
+__global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_cell_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
+}
+
 // CFL dt and |div B| of the current state (prologue; src/solver_func.c:255-343)
 struct DtArgs {
   Grid g;
@@ -912,11 +1172,12 @@ __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o
   return x;
 }
 
-__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+__device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, const int by,
+                                        const int bz) {
   const Grid &g = a.g;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
+  const int k = bx * blockDim.x + threadIdx.x;
+  const int j = by * 4 + threadIdx.y;
+  const int i = bz;
   if (k >= g.N[2] || j >= g.N[1]) return;
   if (i >= g.s[0] && i <= g.e[0] && j >= g.s[1] && j <= g.e[1] && k >= g.s[2] && k <= g.e[2])
     return;
@@ -943,6 +1204,20 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+  k_ghost_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+__global__ __launch_bounds__(256) void k_ghost_batch(const GhostArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_ghost_body(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
+}
+
 // cell-centred B over cells 0..N-2 of each axis (get_derived_variables,
 // src/solver_func.c:310-322)
 struct MagsArgs {
@@ -952,17 +1227,32 @@ struct MagsArgs {
   double *M;
 };
 
-__global__ __launch_bounds__(256) void k_mags(MagsArgs a) {
+__device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
+                                       const int bz) {
   const Grid &g = a.g;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
+  const int k = bx * blockDim.x + threadIdx.x;
+  const int j = by * 4 + threadIdx.y;
+  const int i = bz;
   if (k > g.N[2] - 2 || j > g.N[1] - 2) return;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int idx[3] = {i, j, k};
 #pragma unroll
   for (int m = 0; m < 3; m++)
     a.M[m * g.fs + c] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
+}
+
+__global__ __launch_bounds__(256) void k_mags(MagsArgs a) {
+  k_mags_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: blocks walk a work list of (argument set, block)
+__global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *args, const WorkItem *work,
+                                                    int nwork) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    k_mags_body(args[it.arg], it.bx, it.by, it.bz);
+    __syncthreads();
+  }
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -991,20 +1281,25 @@ struct gk_solver {
   double *X[2];        // 6 fields each: X[cur] = X^n, X[1-cur] = X^{n-1}
   double *B[3];        // face B rotation: B[b_m1], B[b_0] = B^n, B[b_n]
   double *M[3];        // cell-centred B, rotating with B
-  double *F[3];        // 9 face-flux fields per direction
-  double *V[3];        // 2 face-velocity fields per direction
-  double *E;           // 3 edge fields
+  void *F[3];          // 9 face-flux fields per direction (type by precision)
+  void *V[3];          // 2 face-velocity fields per direction
+  void *E;             // 3 edge fields
+  float *tables_f;     // FP32 copies of the reconstruction tables
+  int precision;       // 0 FP64, 1 FP32-bulk
+  size_t esz;          // bytes per intermediate value
   Scal *sc;
   int cur, b_m1, b_0, b_n;
   int ready;
   size_t bytes;
   int timing;
   int chunk;           // faces per thread in the marching sweeps
+  struct gk_batch *batch;  // batch that owns the stepping, if any
   std::vector<std::string> names;
   std::vector<double> acc_ms;
   std::vector<int> acc_n;
   std::vector<hipEvent_t> evpool;  // timing events, reused
   std::vector<int> pend;           // kernel slot per recorded event pair
+  std::vector<struct Launch> plan; // launch list of the current step
 };
 
 #define HIPCHK(x)                                                              \
@@ -1016,11 +1311,15 @@ struct gk_solver {
     }                                                                          \
   } while (0)
 
-static int dev_alloc(gk_solver *s, double **p, size_t ndouble) {
-  if (hipMalloc((void **)p, ndouble * sizeof(double)) != hipSuccess) return GK_ERR_NOMEM;
-  s->bytes += ndouble * sizeof(double);
-  if (hipMemset(*p, 0, ndouble * sizeof(double)) != hipSuccess) return GK_ERR_DEVICE;
+static int dev_alloc_bytes(gk_solver *s, void **p, size_t nbytes) {
+  if (hipMalloc(p, nbytes) != hipSuccess) return GK_ERR_NOMEM;
+  s->bytes += nbytes;
+  if (hipMemset(*p, 0, nbytes) != hipSuccess) return GK_ERR_DEVICE;
   return GK_OK;
+}
+
+static int dev_alloc(gk_solver *s, double **p, size_t ndouble) {
+  return dev_alloc_bytes(s, (void **)p, ndouble * sizeof(double));
 }
 
 extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
@@ -1028,7 +1327,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   *out = nullptr;
   if (d->ni < 5 || d->nj < 5 || d->nk < 5) return GK_ERR_ARG;
   if (d->coord != 0 && d->coord != 2) return GK_ERR_ARG;
-  if (d->precision != 0) return GK_ERR_ARG;
+  if (d->precision != 0 && d->precision != 1) return GK_ERR_ARG;
   if (d->b0_mode != 0 && d->b0_mode != 1) return GK_ERR_ARG;
   for (int q = 0; q < GK_G_COUNT; q++)
     for (int a = 0; a < 3; a++)
@@ -1067,6 +1366,8 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   s->g.CA = d->CA;
   s->fs = s->g.fs;
   s->spherical = d->coord == 2;
+  s->precision = d->precision;
+  s->esz = d->precision == 1 ? sizeof(float) : sizeof(double);
   memcpy(s->bc, d->bc, sizeof(s->bc));
   s->b0 = B0c{};
   if (d->b0_mode == 1) {
@@ -1127,6 +1428,17 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     s->geo.recw[a] = s->tables + off_w[a];
     s->geo.cls[a][0] = s->itables + off_c[a][0];
     s->geo.cls[a][1] = s->itables + off_c[a][1];
+    s->geo.recwf[a] = nullptr;
+  }
+  if (s->precision == 1) {  // FP32 copies of the reconstruction tables
+    std::vector<float> hf(h.size());
+    for (size_t q = 0; q < h.size(); q++) hf[q] = (float)h[q];
+    if ((rc = dev_alloc_bytes(s, (void **)&s->tables_f, hf.size() * sizeof(float))) != GK_OK)
+      return fail(rc);
+    if (hipMemcpy(s->tables_f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail(GK_ERR_DEVICE);
+    for (int a = 0; a < 3; a++) s->geo.recwf[a] = s->tables_f + off_w[a];
   }
 
   const size_t fs = (size_t)s->fs;
@@ -1137,10 +1449,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     if ((rc = dev_alloc(s, &s->M[b], 3 * fs)) != GK_OK) return fail(rc);
   }
   for (int d2 = 0; d2 < 3; d2++) {
-    if ((rc = dev_alloc(s, &s->F[d2], 9 * fs)) != GK_OK) return fail(rc);
-    if ((rc = dev_alloc(s, &s->V[d2], 2 * fs)) != GK_OK) return fail(rc);
+    if ((rc = dev_alloc_bytes(s, &s->F[d2], 9 * fs * s->esz)) != GK_OK) return fail(rc);
+    if ((rc = dev_alloc_bytes(s, &s->V[d2], 2 * fs * s->esz)) != GK_OK) return fail(rc);
   }
-  if ((rc = dev_alloc(s, &s->E, 3 * fs)) != GK_OK) return fail(rc);
+  if ((rc = dev_alloc_bytes(s, &s->E, 3 * fs * s->esz)) != GK_OK) return fail(rc);
   if (hipMalloc((void **)&s->sc, sizeof(Scal)) != hipSuccess) return fail(GK_ERR_NOMEM);
   Scal z{};
   if (hipMemcpy(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice) != hipSuccess)
@@ -1159,8 +1471,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
 
 extern "C" int gk_destroy(gk_solver *s) {
   if (!s) return GK_OK;
+  if (s->batch) return GK_ERR_STATE;  // remove it from its batch first
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->tables);
+  (void)hipFree(s->tables_f);
   (void)hipFree(s->itables);
   for (int b = 0; b < 2; b++) (void)hipFree(s->X[b]);
   for (int b = 0; b < 3; b++) {
@@ -1340,18 +1654,86 @@ void resolve_timers(gk_solver *s) {
 }
 }  // namespace
 
-static void one_step(gk_solver *s) {
+// ---------------------------------------------------------------------------
+// One AB2 step as a list of launches.  The same plan drives the single-grid
+// path (launched one by one) and the batched path (launches of many grids
+// merged per stage into one work-list launch).
+enum StageKernel {
+  SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_EMF_I, SK_EMF_J, SK_EMF_K,
+  SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
+};
+static const char *const sk_name[SK_COUNT] = {
+    "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_emf_i",
+    "k_emf_j", "k_emf_k", "k_ct", "k_cell", "k_ghost", "k_mags"};
+
+union StageArgs {
+  SweepArgs sw;
+  EmfArgs em;
+  CtArgs ct;
+  CellArgs ce;
+  GhostArgs gh;
+  MagsArgs mg;
+  Scal *sc;
+};
+
+static size_t stage_arg_size(int kid) {
+  switch (kid) {
+    case SK_BEGIN: return sizeof(Scal *);
+    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: return sizeof(SweepArgs);
+    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return sizeof(EmfArgs);
+    case SK_CT: return sizeof(CtArgs);
+    case SK_CELL: return sizeof(CellArgs);
+    case SK_GHOST: return sizeof(GhostArgs);
+    default: return sizeof(MagsArgs);
+  }
+}
+
+static const void *stage_arg_ptr(int kid, const StageArgs &u) {
+  switch (kid) {
+    case SK_BEGIN: return &u.sc;
+    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: return &u.sw;
+    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return &u.em;
+    case SK_CT: return &u.ct;
+    case SK_CELL: return &u.ce;
+    case SK_GHOST: return &u.gh;
+    default: return &u.mg;
+  }
+}
+
+struct Launch {
+  int kid;
+  dim3 grid;
+  StageArgs u;
+};
+
+// history rotation: X[cur] = X^n, X[1-cur] = X^{n-1}; B/M[b_m1], [b_0], [b_n]
+struct Rot {
+  int cur, b_m1, b_0, b_n;
+};
+static Rot rot_of(const gk_solver *s) { return Rot{s->cur, s->b_m1, s->b_0, s->b_n}; }
+static Rot rot_next(Rot r) { return Rot{1 - r.cur, r.b_0, r.b_n, r.b_m1}; }
+
+static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out) {
+  out.clear();
   const Grid &g = s->g;
   const int *S = g.s, *Ee = g.e;
-  const double *X = s->X[s->cur], *Xp = s->X[1 - s->cur];
-  const double *Bm1 = s->B[s->b_m1], *B = s->B[s->b_0];
-  const double *Mm1 = s->M[s->b_m1], *M = s->M[s->b_0];
-  double *Bn = s->B[s->b_n], *Mn = s->M[s->b_n];
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, s->stream, s->sc);
+  const double *X = s->X[r.cur], *Xp = s->X[1 - r.cur];
+  const double *Bm1 = s->B[r.b_m1], *B = s->B[r.b_0];
+  const double *Mm1 = s->M[r.b_m1], *M = s->M[r.b_0];
+  double *Bn = s->B[r.b_n], *Mn = s->M[r.b_n];
+  double *Xn = s->X[1 - r.cur];
+  const int ch = s->chunk;
+  Launch L;
+  memset(&L, 0, sizeof L);
+  L.kid = SK_BEGIN;
+  L.grid = dim3(1, 1, 1);
+  L.u.sc = s->sc;
+  out.push_back(L);
 
   // directional sweeps: face fluxes + face velocities
   for (int D = 0; D < 3; D++) {
     SweepArgs a;
+    memset(&a, 0, sizeof a);
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
@@ -1363,6 +1745,7 @@ static void one_step(gk_solver *s) {
     a.F = s->F[D];
     a.V = s->V[D];
     a.sc = s->sc;
+    a.chunk = ch;
     for (int d = 0; d < 3; d++) {
       a.lo[d] = S[d];
       a.hi[d] = Ee[d];
@@ -1376,37 +1759,44 @@ static void one_step(gk_solver *s) {
     const int X2 = D == 0 ? 1 : (D == 1 ? 2 : 0);
     a.lo[X2] = 0;
     a.hi[X2] = g.N[X2] - 2;
-    static const char *nm[3] = {"k_sweep_i", "k_sweep_j", "k_sweep_k"};
-    Timer t(s, nm[D]);
-    const int ch = s->chunk;
     if (D == 0) {
-      dim3 grid((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + 4) / 4,
-                (a.hi[0] - a.lo[0] + ch) / ch);
-      hipLaunchKernelGGL(k_march<0>, grid, dim3(64, 4), 0, s->stream, a, ch);
+      L.kid = SK_SWEEP_I;
+      L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + 4) / 4,
+                    (a.hi[0] - a.lo[0] + ch) / ch);
+      L.u.sw = a;
+      out.push_back(L);
     } else if (D == 1) {
       SweepArgs m = a;  // full faces: k over the active range
       m.lo[2] = S[2];
       m.hi[2] = Ee[2];
-      dim3 grid((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[0] - m.lo[0] + 4) / 4,
-                (m.hi[1] - m.lo[1] + ch) / ch);
-      hipLaunchKernelGGL(k_march<1>, grid, dim3(64, 4), 0, s->stream, m, ch);
+      L.kid = SK_SWEEP_J;
+      L.grid = dim3((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[0] - m.lo[0] + 4) / 4,
+                    (m.hi[1] - m.lo[1] + ch) / ch);
+      L.u.sw = m;
+      out.push_back(L);
       // velocity-only strips k in [0, ks-1] and [ke+1, NK-2]
       for (int side = 0; side < 2; side++) {
         SweepArgs v = a;
         v.lo[2] = side == 0 ? 0 : Ee[2] + 1;
         v.hi[2] = side == 0 ? S[2] - 1 : g.N[2] - 2;
-        dim3 gv((v.hi[2] - v.lo[2] + 64) / 64, (v.hi[1] - v.lo[1] + 4) / 4, v.hi[0] - v.lo[0] + 1);
-        hipLaunchKernelGGL(k_sweep<1>, gv, dim3(64, 4), 0, s->stream, v);
+        L.kid = SK_VSTRIP_J;
+        L.grid = dim3((v.hi[2] - v.lo[2] + 64) / 64, (v.hi[1] - v.lo[1] + 4) / 4,
+                      v.hi[0] - v.lo[0] + 1);
+        L.u.sw = v;
+        out.push_back(L);
       }
     } else {
-      dim3 grid((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + KROWS) / KROWS,
-                a.hi[0] - a.lo[0] + 1);
-      hipLaunchKernelGGL(k_sweepk, grid, dim3(64, KROWS), 0, s->stream, a);
+      L.kid = SK_SWEEP_K;
+      L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + KROWS) / KROWS,
+                    a.hi[0] - a.lo[0] + 1);
+      L.u.sw = a;
+      out.push_back(L);
     }
   }
   // edge electric fields
   for (int D = 0; D < 3; D++) {
     EmfArgs a;
+    memset(&a, 0, sizeof a);
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
@@ -1416,35 +1806,37 @@ static void one_step(gk_solver *s) {
     a.Bp = Bm1;
     // E_i uses the j-sweep velocities, E_j the k-sweep, E_k the i-sweep
     a.V = s->V[D == 0 ? 1 : (D == 1 ? 2 : 0)];
-    a.E = s->E + D * s->fs;
+    a.E = (char *)s->E + (size_t)D * s->fs * s->esz;
     a.sc = s->sc;
+    a.chunk = ch;
     for (int d = 0; d < 3; d++) {
       a.lo[d] = S[d];
       a.hi[d] = Ee[d] + (d == D ? 0 : 1);
     }
-    dim3 grid(((a.hi[2] - a.lo[2] + 1) + 63) / 64, ((a.hi[1] - a.lo[1] + 1) + 3) / 4,
-              a.hi[0] - a.lo[0] + 1);
-    static const char *nm[3] = {"k_emf_i", "k_emf_j", "k_emf_k"};
-    Timer t(s, nm[D]);
-    if (D == 0) hipLaunchKernelGGL(k_emf<0>, grid, dim3(64, 4), 0, s->stream, a);
-    if (D == 1) hipLaunchKernelGGL(k_emf<1>, grid, dim3(64, 4), 0, s->stream, a);
-    if (D == 2) hipLaunchKernelGGL(k_emf<2>, grid, dim3(64, 4), 0, s->stream, a);
+    const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
+    L.kid = SK_EMF_I + D;
+    L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[RA] - a.lo[RA] + 4) / 4,
+                  (a.hi[MA] - a.lo[MA] + ch) / ch);
+    L.u.em = a;
+    out.push_back(L);
   }
   {
     CtArgs a;
+    memset(&a, 0, sizeof a);
     a.g = g;
     a.geo = s->geo;
     a.B = B;
     a.Bn = Bn;
     a.E = s->E;
     a.sc = s->sc;
-    Timer t(s, "k_ct");
-    hipLaunchKernelGGL(k_ct, box_grid(s->nk + 1, s->nj + 1, s->ni + 1), dim3(64, 4), 0, s->stream,
-                       a);
+    L.kid = SK_CT;
+    L.grid = box_grid(s->nk + 1, s->nj + 1, s->ni + 1);
+    L.u.ct = a;
+    out.push_back(L);
   }
-  double *Xn = s->X[1 - s->cur];
   {
     CellArgs a;
+    memset(&a, 0, sizeof a);
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
@@ -1457,28 +1849,96 @@ static void one_step(gk_solver *s) {
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
-    Timer t(s, "k_cell");
-    hipLaunchKernelGGL(k_cell, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
+    L.kid = SK_CELL;
+    L.grid = box_grid(s->nk, s->nj, s->ni);
+    L.u.ce = a;
+    out.push_back(L);
   }
   {
-    Timer t(s, "k_ghost");
-    launch_ghost(s, Xn, Bn);
+    GhostArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = g;
+    a.X = Xn;
+    a.B = Bn;
+    memcpy(a.bc, s->bc, sizeof(a.bc));
+    L.kid = SK_GHOST;
+    L.grid = box_grid(g.N[2], g.N[1], g.N[0]);
+    L.u.gh = a;
+    out.push_back(L);
   }
   {
-    Timer t(s, "k_mags");
-    launch_mags(s, Bn, Mn);
+    MagsArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = g;
+    a.geo = s->geo;
+    a.B = Bn;
+    a.M = Mn;
+    L.kid = SK_MAGS;
+    L.grid = box_grid(g.N[2] - 1, g.N[1] - 1, g.N[0] - 1);
+    L.u.mg = a;
+    out.push_back(L);
   }
-  // rotate history
-  s->cur = 1 - s->cur;
-  int old_m1 = s->b_m1;
-  s->b_m1 = s->b_0;
-  s->b_0 = s->b_n;
-  s->b_n = old_m1;
+}
+
+template <typename R>
+static void launch_one(const Launch &L, hipStream_t st) {
+  const dim3 blk(64, 4);
+  switch (L.kid) {
+    case SK_BEGIN: k_begin<<<1, 1, 0, st>>>(L.u.sc); break;
+    case SK_SWEEP_I: k_march<0, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
+    case SK_SWEEP_J: k_march<1, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
+    case SK_VSTRIP_J: k_sweep<1, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
+    case SK_SWEEP_K: k_sweepk<R><<<L.grid, dim3(64, KROWS), 0, st>>>(L.u.sw); break;
+    case SK_EMF_I: k_emf_march<0, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
+    case SK_EMF_J: k_emf_march<1, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
+    case SK_EMF_K: k_emf_march<2, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
+    case SK_CT: k_ct<R><<<L.grid, blk, 0, st>>>(L.u.ct); break;
+    case SK_CELL: k_cell<R><<<L.grid, blk, 0, st>>>(L.u.ce); break;
+    case SK_GHOST: k_ghost<<<L.grid, blk, 0, st>>>(L.u.gh); break;
+    case SK_MAGS: k_mags<<<L.grid, blk, 0, st>>>(L.u.mg); break;
+  }
+}
+
+template <typename R>
+static void launch_batch(int kid, const void *args, const WorkItem *work, int nwork, int nblocks,
+                         hipStream_t st) {
+  const dim3 blk(64, 4);
+  switch (kid) {
+    case SK_BEGIN: k_begin_batch<<<(nwork + 63) / 64, 64, 0, st>>>((Scal *const *)args, nwork); break;
+    case SK_SWEEP_I: k_march_batch<0, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64, KROWS), 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_EMF_I: k_emf_march_batch<0, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_J: k_emf_march_batch<1, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_K: k_emf_march_batch<2, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_CT: k_ct_batch<R><<<nblocks, blk, 0, st>>>((const CtArgs *)args, work, nwork); break;
+    case SK_CELL: k_cell_batch<R><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
+    case SK_GHOST: k_ghost_batch<<<nblocks, blk, 0, st>>>((const GhostArgs *)args, work, nwork); break;
+    case SK_MAGS: k_mags_batch<<<nblocks, blk, 0, st>>>((const MagsArgs *)args, work, nwork); break;
+  }
+}
+
+static void one_step(gk_solver *s) {
+  std::vector<Launch> &plan = s->plan;
+  plan_step(s, rot_of(s), plan);
+  for (const Launch &L : plan) {
+    Timer t(s, sk_name[L.kid]);
+    if (s->precision == 1)
+      launch_one<float>(L, s->stream);
+    else
+      launch_one<double>(L, s->stream);
+  }
+  const Rot r = rot_next(rot_of(s));
+  s->cur = r.cur;
+  s->b_m1 = r.b_m1;
+  s->b_0 = r.b_0;
+  s->b_n = r.b_n;
 }
 
 extern "C" int gk_step(gk_solver *s, int n) {
   if (!s) return GK_ERR_ARG;
-  if (s->ready != 3) return GK_ERR_STATE;
+  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
   if (n < 0) return GK_ERR_ARG;
   if (s->timing)
     for (size_t q = 0; q < s->acc_ms.size(); q++) {
@@ -1521,15 +1981,33 @@ extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], dou
 
 extern "C" int gk_debug_fetch(gk_solver *s, int which, int field, double *host) {
   if (!s || !host || field < 0) return GK_ERR_ARG;
-  const double *src = nullptr;
-  if (which >= 0 && which < 3 && field < 9) src = s->F[which] + field * s->fs;
-  if (which >= 3 && which < 6 && field < 2) src = s->V[which - 3] + field * s->fs;
-  if (which == 6 && field < 3) src = s->E + field * s->fs;
-  if (which == 7 && field < 3) src = s->M[s->b_0] + field * s->fs;
+  const char *src = nullptr;
+  size_t esz = s->esz;
+  const size_t fb = (size_t)s->fs * s->esz;
+  if (which >= 0 && which < 3 && field < 9) src = (const char *)s->F[which] + field * fb;
+  if (which >= 3 && which < 6 && field < 2) src = (const char *)s->V[which - 3] + field * fb;
+  if (which == 6 && field < 3) src = (const char *)s->E + field * fb;
+  if (which == 7 && field < 3) {
+    src = (const char *)(s->M[s->b_0] + field * s->fs);
+    esz = sizeof(double);
+  }
   if (!src) return GK_ERR_ARG;
-  int rc = copy_out(s, host, src);
-  if (rc != GK_OK) return rc;
+  if (esz == sizeof(double)) {
+    int rc = copy_out(s, host, (const double *)src);
+    if (rc != GK_OK) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GK_OK;
+  }
+  // FP32 intermediates: copy the padded block, widen into the host layout
+  std::vector<float> tmp((size_t)s->fs);
+  HIPCHK(hipMemcpyAsync(tmp.data(), src, (size_t)s->fs * sizeof(float), hipMemcpyDeviceToHost,
+                        s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  const size_t NK = s->g.N[2];
+  for (int i = 0; i < s->g.N[0]; i++)
+    for (int j = 0; j < s->g.N[1]; j++)
+      for (size_t k = 0; k < NK; k++)
+        host[((size_t)i * s->g.N[1] + j) * NK + k] = tmp[i * s->g.st[0] + j * s->g.st[1] + k];
   return GK_OK;
 }
 
@@ -1619,4 +2097,380 @@ extern "C" int gk_get_tuning(const gk_solver *s, int param) {
   if (!s) return GK_ERR_ARG;
   if (param == GK_TUNE_CHUNK) return s->chunk;
   return GK_ERR_ARG;
+}
+
+// ===========================================================================
+// Batched stepping of many grids: per stage, one launch whose blocks walk a
+// work list covering every grid in the batch (grids of one precision share a
+// launch); the launches of a step are captured in HIP graphs.  Each grid keeps
+// its own dt, time and history, and runs exactly the arithmetic of gk_step.
+struct gk_batch {
+  std::vector<gk_solver *> mem;
+  hipStream_t stream = nullptr;
+  bool dirty = true;
+  int phase = 0;  // batch steps since the tables were built, mod 6
+  char *dargs = nullptr;
+  WorkItem *dwork = nullptr;
+  size_t args_off[2][6][SK_COUNT];
+  int nargs[2][SK_COUNT];
+  size_t work_off[2][SK_COUNT];
+  int nwork[2][SK_COUNT];
+  int use_graphs = 1;
+  hipGraphExec_t g1[6] = {};
+  hipGraphExec_t g6 = nullptr;
+  int timing = 0;
+  std::vector<std::string> names;
+  std::vector<double> acc_ms;
+  std::vector<int> acc_n;
+};
+
+static void batch_free_tables(gk_batch *b) {
+  for (int p = 0; p < 6; p++)
+    if (b->g1[p]) {
+      (void)hipGraphExecDestroy(b->g1[p]);
+      b->g1[p] = nullptr;
+    }
+  if (b->g6) {
+    (void)hipGraphExecDestroy(b->g6);
+    b->g6 = nullptr;
+  }
+  (void)hipFree(b->dargs);
+  (void)hipFree(b->dwork);
+  b->dargs = nullptr;
+  b->dwork = nullptr;
+}
+
+static int batch_build(gk_batch *b) {
+  batch_free_tables(b);
+  std::vector<char> hargs[2][6][SK_COUNT];
+  std::vector<WorkItem> hwork[2][SK_COUNT];
+  std::vector<Launch> plan;
+  for (gk_solver *s : b->mem) {
+    const int gp = s->precision == 1 ? 1 : 0;
+    Rot r = rot_of(s);
+    for (int p = 0; p < 6; p++) {
+      plan_step(s, r, plan);
+      for (const Launch &L : plan) {
+        const size_t sz = stage_arg_size(L.kid);
+        std::vector<char> &A = hargs[gp][p][L.kid];
+        const int idx = (int)(A.size() / sz);
+        const char *src = (const char *)stage_arg_ptr(L.kid, L.u);
+        A.insert(A.end(), src, src + sz);
+        if (p == 0 && L.kid != SK_BEGIN)
+          for (unsigned z = 0; z < L.grid.z; z++)
+            for (unsigned y = 0; y < L.grid.y; y++)
+              for (unsigned x = 0; x < L.grid.x; x++)
+                hwork[gp][L.kid].push_back(WorkItem{idx, (int)x, (int)y, (int)z});
+      }
+      r = rot_next(r);
+    }
+  }
+  size_t abytes = 0, nw = 0;
+  for (int gp = 0; gp < 2; gp++)
+    for (int k = 0; k < SK_COUNT; k++) {
+      for (int p = 0; p < 6; p++) {
+        b->args_off[gp][p][k] = abytes;
+        abytes += (hargs[gp][p][k].size() + 255) / 256 * 256;
+      }
+      b->nargs[gp][k] = (int)(hargs[gp][0][k].size() / stage_arg_size(k));
+      b->work_off[gp][k] = nw;
+      b->nwork[gp][k] = k == SK_BEGIN ? b->nargs[gp][k] : (int)hwork[gp][k].size();
+      nw += hwork[gp][k].size();
+    }
+  if (abytes && hipMalloc((void **)&b->dargs, abytes) != hipSuccess) return GK_ERR_NOMEM;
+  if (nw && hipMalloc((void **)&b->dwork, nw * sizeof(WorkItem)) != hipSuccess) return GK_ERR_NOMEM;
+  for (int gp = 0; gp < 2; gp++)
+    for (int k = 0; k < SK_COUNT; k++) {
+      for (int p = 0; p < 6; p++)
+        if (!hargs[gp][p][k].empty())
+          HIPCHK(hipMemcpy(b->dargs + b->args_off[gp][p][k], hargs[gp][p][k].data(),
+                           hargs[gp][p][k].size(), hipMemcpyHostToDevice));
+      if (!hwork[gp][k].empty())
+        HIPCHK(hipMemcpy(b->dwork + b->work_off[gp][k], hwork[gp][k].data(),
+                         hwork[gp][k].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+  b->phase = 0;
+  b->dirty = false;
+  return GK_OK;
+}
+
+// launches of one batch step at phase p
+static void batch_launch_step(gk_batch *b, int p, bool timed) {
+  for (int k = 0; k < SK_COUNT; k++)
+    for (int gp = 0; gp < 2; gp++) {
+      const int nw = b->nwork[gp][k];
+      if (!nw) continue;
+      const int nblocks = nw < 4096 ? nw : 4096;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (timed) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, b->stream);
+      }
+      const void *args = b->dargs + b->args_off[gp][p][k];
+      const WorkItem *work = b->dwork + b->work_off[gp][k];
+      if (gp == 1)
+        launch_batch<float>(k, args, work, nw, nblocks, b->stream);
+      else
+        launch_batch<double>(k, args, work, nw, nblocks, b->stream);
+      if (timed) {
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        std::string nm = std::string(sk_name[k]) + (gp ? "/f32" : "/f64");
+        int slot = -1;
+        for (size_t q = 0; q < b->names.size(); q++)
+          if (b->names[q] == nm) slot = (int)q;
+        if (slot < 0) {
+          slot = (int)b->names.size();
+          b->names.push_back(nm);
+          b->acc_ms.push_back(0.0);
+          b->acc_n.push_back(0);
+        }
+        b->acc_ms[slot] += ms;
+        b->acc_n[slot] += 1;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+      }
+    }
+}
+
+static int batch_capture(gk_batch *b, int first_phase, int nsteps, hipGraphExec_t *out) {
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int t = 0; t < nsteps; t++) batch_launch_step(b, (first_phase + t) % 6, false);
+  HIPCHK(hipStreamEndCapture(b->stream, &graph));
+  hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return GK_ERR_DEVICE;
+  }
+  return GK_OK;
+}
+
+extern "C" int gk_batch_create(gk_batch **out) {
+  if (!out) return GK_ERR_ARG;
+  gk_batch *b = new gk_batch();
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return GK_ERR_DEVICE;
+  }
+  *out = b;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_destroy(gk_batch *b) {
+  if (!b) return GK_OK;
+  (void)hipStreamSynchronize(b->stream);
+  for (gk_solver *s : b->mem) s->batch = nullptr;
+  batch_free_tables(b);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_add(gk_batch *b, gk_solver *s) {
+  if (!b || !s) return GK_ERR_ARG;
+  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  b->mem.push_back(s);
+  s->batch = b;
+  b->dirty = true;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_remove(gk_batch *b, gk_solver *s) {
+  if (!b || !s) return GK_ERR_ARG;
+  for (size_t q = 0; q < b->mem.size(); q++)
+    if (b->mem[q] == s) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      b->mem.erase(b->mem.begin() + q);
+      s->batch = nullptr;
+      b->dirty = true;
+      return GK_OK;
+    }
+  return GK_ERR_ARG;
+}
+
+extern "C" int gk_batch_size(const gk_batch *b) { return b ? (int)b->mem.size() : GK_ERR_ARG; }
+
+extern "C" int gk_batch_set_graphs(gk_batch *b, int on) {
+  if (!b) return GK_ERR_ARG;
+  b->use_graphs = on ? 1 : 0;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_enable_timing(gk_batch *b, int on) {
+  if (!b) return GK_ERR_ARG;
+  b->timing = on ? 1 : 0;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_kernel_times(gk_batch *b, const char **names, double *ms, int *n) {
+  if (!b || !n) return GK_ERR_ARG;
+  int cnt = (int)b->names.size();
+  int m = *n < cnt ? *n : cnt;
+  for (int q = 0; q < m; q++) {
+    if (names) names[q] = b->names[q].c_str();
+    if (ms) ms[q] = b->acc_n[q] ? b->acc_ms[q] / b->acc_n[q] : 0.0;
+  }
+  *n = cnt;
+  return GK_OK;
+}
+
+// Launch n batch steps without waiting for them (graphs unless disabled or
+// timing); gk_batch_sync waits.
+extern "C" int gk_batch_step_async(gk_batch *b, int n) {
+  if (!b || n < 0) return GK_ERR_ARG;
+  if (b->mem.empty() || n == 0) return GK_OK;
+  if (b->dirty) {
+    int rc = batch_build(b);
+    if (rc != GK_OK) return rc;
+  }
+  if (b->timing) {
+    for (size_t q = 0; q < b->acc_ms.size(); q++) {
+      b->acc_ms[q] = 0.0;
+      b->acc_n[q] = 0;
+    }
+  }
+  int left = n;
+  while (left > 0) {
+    if (b->timing || !b->use_graphs) {
+      batch_launch_step(b, b->phase, b->timing != 0);
+      left -= 1;
+      b->phase = (b->phase + 1) % 6;
+      continue;
+    }
+    if (b->phase == 0 && left >= 6) {
+      if (!b->g6) {
+        int rc = batch_capture(b, 0, 6, &b->g6);
+        if (rc != GK_OK) return rc;
+      }
+      HIPCHK(hipGraphLaunch(b->g6, b->stream));
+      left -= 6;
+    } else {
+      if (!b->g1[b->phase]) {
+        int rc = batch_capture(b, b->phase, 1, &b->g1[b->phase]);
+        if (rc != GK_OK) return rc;
+      }
+      HIPCHK(hipGraphLaunch(b->g1[b->phase], b->stream));
+      left -= 1;
+      b->phase = (b->phase + 1) % 6;
+    }
+  }
+  // every grid's history rotation advances by n
+  for (gk_solver *s : b->mem) {
+    Rot r = rot_of(s);
+    for (int t = 0; t < n % 6; t++) r = rot_next(r);
+    s->cur = r.cur;
+    s->b_m1 = r.b_m1;
+    s->b_0 = r.b_0;
+    s->b_n = r.b_n;
+  }
+  HIPCHK(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_batch_sync(gk_batch *b) {
+  if (!b) return GK_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return GK_OK;
+}
+
+extern "C" int gk_batch_step(gk_batch *b, int n) {
+  int rc = gk_batch_step_async(b, n);
+  if (rc != GK_OK) return rc;
+  return gk_batch_sync(b);
+}
+
+// Launch-shape autotuner: times `trial_steps` steps for each candidate march
+// length on this grid and keeps the fastest.  The state (both time levels,
+// face and cell B, scalars, history rotation) is saved first and restored
+// afterwards, so results do not depend on tuning.  Picks are cached per
+// (ni, nj, nk, coord, precision) for the life of the process.
+struct TuneKey {
+  int ni, nj, nk, coord, prec;
+  bool operator==(const TuneKey &o) const {
+    return ni == o.ni && nj == o.nj && nk == o.nk && coord == o.coord && prec == o.prec;
+  }
+};
+static std::vector<std::pair<TuneKey, int>> g_tune_cache;
+
+extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
+  if (!s || trial_steps < 1) return GK_ERR_ARG;
+  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
+  const TuneKey key{s->ni, s->nj, s->nk, s->spherical, s->precision};
+  for (auto &e : g_tune_cache)
+    if (e.first == key) {
+      s->chunk = e.second;
+      if (best) *best = e.second;
+      return GK_OK;
+    }
+  const size_t fs = s->fs;
+  const size_t nX = 12 * fs, nB = 9 * fs, nM = 9 * fs;
+  double *save = nullptr;
+  if (hipMalloc((void **)&save, (nX + nB + nM) * sizeof(double) + sizeof(Scal)) != hipSuccess)
+    return GK_ERR_NOMEM;
+  const Rot r0 = rot_of(s);
+  auto snap = [&](bool restore) -> int {
+    double *p = save;
+    const hipMemcpyKind kd = hipMemcpyDeviceToDevice;
+    for (int q = 0; q < 2; q++, p += 6 * fs)
+      HIPCHK(restore ? hipMemcpyAsync(s->X[q], p, 6 * fs * 8, kd, s->stream)
+                     : hipMemcpyAsync(p, s->X[q], 6 * fs * 8, kd, s->stream));
+    for (int q = 0; q < 3; q++, p += 3 * fs)
+      HIPCHK(restore ? hipMemcpyAsync(s->B[q], p, 3 * fs * 8, kd, s->stream)
+                     : hipMemcpyAsync(p, s->B[q], 3 * fs * 8, kd, s->stream));
+    for (int q = 0; q < 3; q++, p += 3 * fs)
+      HIPCHK(restore ? hipMemcpyAsync(s->M[q], p, 3 * fs * 8, kd, s->stream)
+                     : hipMemcpyAsync(p, s->M[q], 3 * fs * 8, kd, s->stream));
+    HIPCHK(restore ? hipMemcpyAsync(s->sc, p, sizeof(Scal), kd, s->stream)
+                   : hipMemcpyAsync(p, s->sc, sizeof(Scal), kd, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->cur = r0.cur;
+    s->b_m1 = r0.b_m1;
+    s->b_0 = r0.b_0;
+    s->b_n = r0.b_n;
+    return GK_OK;
+  };
+  int rc = snap(false);
+  const int maxlen = std::max(s->ni, std::max(s->nj, s->nk)) + 1;
+  const int cand[] = {8, 16, 32, 64, 128, 256};
+  int pick = s->chunk;
+  float best_ms = 1e30f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == GK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = GK_ERR_DEVICE;
+  const int keep_timing = s->timing;
+  s->timing = 0;
+  for (int c : cand) {
+    if (rc != GK_OK) break;
+    if (c > 2 * maxlen && c != cand[0]) break;
+    s->chunk = c;
+    one_step(s);  // warm-up (first launch of this shape)
+    (void)hipEventRecord(e0, s->stream);
+    for (int t = 0; t < trial_steps; t++) one_step(s);
+    (void)hipEventRecord(e1, s->stream);
+    float ms = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = GK_ERR_DEVICE;
+      break;
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      pick = c;
+    }
+    rc = snap(true);
+  }
+  s->timing = keep_timing;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(save);
+  if (rc != GK_OK) return rc;
+  s->chunk = pick;
+  g_tune_cache.push_back({key, pick});
+  if (best) *best = pick;
+  return GK_OK;
 }

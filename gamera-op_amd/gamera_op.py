@@ -71,6 +71,28 @@ def _load():
     _op.gk_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     _op.gk_get_tuning.restype = ctypes.c_int
     _op.gk_get_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    vp = ctypes.c_void_p
+    h.gkh_device_create.restype = ctypes.c_int
+    h.gkh_device_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    for name, args in [("gk_autotune", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+                       ("gk_destroy", [vp]),
+                       ("gk_get_state", [vp] + [ctypes.POINTER(ctypes.c_void_p)] * 4),
+                       ("gk_batch_create", [ctypes.POINTER(vp)]),
+                       ("gk_batch_destroy", [vp]),
+                       ("gk_batch_add", [vp, vp]),
+                       ("gk_batch_remove", [vp, vp]),
+                       ("gk_batch_size", [vp]),
+                       ("gk_batch_step", [vp, ctypes.c_int]),
+                       ("gk_batch_step_async", [vp, ctypes.c_int]),
+                       ("gk_batch_sync", [vp]),
+                       ("gk_batch_set_graphs", [vp, ctypes.c_int]),
+                       ("gk_batch_enable_timing", [vp, ctypes.c_int]),
+                       ("gk_batch_kernel_times", [vp, ctypes.POINTER(ctypes.c_char_p),
+                                                  ctypes.POINTER(ctypes.c_double),
+                                                  ctypes.POINTER(ctypes.c_int)])]:
+        f = getattr(_op, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
     _op.gk_copy_calibrate.restype = ctypes.c_int
     _op.gk_copy_calibrate.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _host = h
@@ -209,3 +231,105 @@ class Problem:
 
     def close(self):
         self.h.finalize_solver()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise GkError(f"{what} failed ({rc})")
+
+
+class Solver:
+    """A standalone device solver for the problem the host facade currently
+    holds (created after Problem(...)); it keeps its own device state, so the
+    next Problem can be set up while this one runs."""
+
+    def __init__(self, problem, precision=0):
+        self.h, self.op = _load()
+        p = ctypes.c_void_p()
+        _check(self.h.gkh_device_create(precision, ctypes.byref(p)), "gkh_device_create")
+        self.handle = p
+        self.n, self.shape = problem.n, problem.shape
+        self.precision = precision
+
+    def step(self, n):
+        _check(self.op.gk_step(self.handle, n), "gk_step")
+
+    def scalars(self):
+        out = (ctypes.c_double * 6)()
+        _check(self.op.gk_get_scalars(self.handle, out), "gk_get_scalars")
+        return dict(dt=out[0], dt0=out[1], time=out[2], divB=out[3], next_dt=out[4],
+                    nonfinite=out[5])
+
+    def state(self):
+        """Primitives (6) and face B (3) in the reference layout."""
+        prim = [np.zeros(self.shape) for _ in range(6)]
+        bf = [np.zeros(self.shape) for _ in range(3)]
+        P = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in prim])
+        B = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in bf])
+        _check(self.op.gk_get_state(self.handle, P, B, None, None), "gk_get_state")
+        return prim, bf
+
+    def autotune(self, trial_steps=3):
+        best = ctypes.c_int(0)
+        _check(self.op.gk_autotune(self.handle, trial_steps, ctypes.byref(best)), "gk_autotune")
+        return best.value
+
+    def set_tuning(self, param, value):
+        _check(self.op.gk_set_tuning(self.handle, param, value), "gk_set_tuning")
+
+    def get_tuning(self, param):
+        return self.op.gk_get_tuning(self.handle, param)
+
+    def close(self):
+        if self.handle:
+            _check(self.op.gk_destroy(self.handle), "gk_destroy")
+            self.handle = None
+
+
+class Batch:
+    """Steps many Solvers together (one launch per stage for all of them,
+    captured in HIP graphs)."""
+
+    def __init__(self):
+        self.h, self.op = _load()
+        p = ctypes.c_void_p()
+        _check(self.op.gk_batch_create(ctypes.byref(p)), "gk_batch_create")
+        self.handle = p
+        self.members = []
+
+    def add(self, solver):
+        _check(self.op.gk_batch_add(self.handle, solver.handle), "gk_batch_add")
+        self.members.append(solver)
+
+    def remove(self, solver):
+        _check(self.op.gk_batch_remove(self.handle, solver.handle), "gk_batch_remove")
+        self.members.remove(solver)
+
+    def step(self, n):
+        _check(self.op.gk_batch_step(self.handle, n), "gk_batch_step")
+
+    def step_async(self, n):
+        _check(self.op.gk_batch_step_async(self.handle, n), "gk_batch_step_async")
+
+    def sync(self):
+        _check(self.op.gk_batch_sync(self.handle), "gk_batch_sync")
+
+    def set_graphs(self, on):
+        _check(self.op.gk_batch_set_graphs(self.handle, 1 if on else 0), "gk_batch_set_graphs")
+
+    def enable_timing(self, on=True):
+        _check(self.op.gk_batch_enable_timing(self.handle, 1 if on else 0), "gk_batch_enable_timing")
+
+    def kernel_times(self):
+        n = ctypes.c_int(64)
+        names = (ctypes.c_char_p * 64)()
+        ms = (ctypes.c_double * 64)()
+        self.op.gk_batch_kernel_times(self.handle, names, ms, ctypes.byref(n))
+        return {names[q].decode(): ms[q] for q in range(min(n.value, 64))}
+
+    def close(self):
+        if self.handle:
+            for m in list(self.members):
+                self.remove(m)
+            _check(self.op.gk_batch_destroy(self.handle), "gk_batch_destroy")
+            self.handle = None

@@ -214,6 +214,35 @@ int gkh_device_open(int precision) {
 
 gk_solver *gkh_device(void) { return g_dev; }
 
+/* A standalone device solver for the problem currently held by the host
+   facade (geometry, weights, B0, initial state): the caller owns it and may
+   set up the next problem right away (batched ensembles, serving). */
+int gkh_device_create(int precision, gk_solver **out) {
+  if (!out) return GK_ERR_ARG;
+  *out = NULL;
+  record_bcs();
+  gk_tables t;
+  if (gk_tables_build(&t) != 0) return GK_ERR_ARG;
+  gk_grid_desc d;
+  int rc = gk_fill_desc(&d, &t, precision);
+  gk_solver *s = NULL;
+  if (rc == GK_OK) rc = gk_create(&d, &s);
+  gk_tables_free(&t);
+  if (rc != GK_OK) return rc;
+  const double *prim[6], *bf[3];
+  for (int m = 0; m < 6; m++) prim[m] = &gas[0][m][0][0][0];
+  for (int m = 0; m < 3; m++) bf[m] = &gem[m][0][0][0];
+  rc = gk_set_state(s, prim, bf, NULL, NULL);
+  if (rc == GK_OK) rc = gk_prologue(s, 0.0);
+  if (rc == GK_OK) rc = gk_set_time(s, time_sim);
+  if (rc != GK_OK) {
+    gk_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return GK_OK;
+}
+
 int gkh_device_fetch(void) {
   if (!g_dev) return GK_ERR_STATE;
   double *prim[6], *bf[3], *pp[6], *bp[3];

@@ -131,6 +131,33 @@ enum { GK_TUNE_CHUNK = 0 };
 int gk_set_tuning(gk_solver *s, int param, int value);
 int gk_get_tuning(const gk_solver *s, int param);
 
+/* Launch-shape autotuner (no counterpart in the reference): times
+   trial_steps steps for each candidate march length on this grid, keeps the
+   fastest (written to *best) and restores the state it started from, so
+   results do not depend on tuning.  Picks are cached per grid shape and
+   precision for the life of the process. */
+int gk_autotune(gk_solver *s, int trial_steps, int *best);
+
+/* Batched stepping (no counterpart in the reference; for ensembles and the
+   serving loop).  A batch steps many prepared solvers (after gk_prologue)
+   together: per stage one launch covers every grid of a precision, and the
+   launches of a step are captured in HIP graphs.  Each grid keeps its own dt,
+   time and history and gets exactly the results gk_step would give it.
+   While a solver is in a batch, gk_step on it returns GK_ERR_STATE;
+   gk_get_state / gk_get_scalars work after gk_batch_step (or gk_batch_sync). */
+typedef struct gk_batch gk_batch;
+int gk_batch_create(gk_batch **out);
+int gk_batch_destroy(gk_batch *b);
+int gk_batch_add(gk_batch *b, gk_solver *s);
+int gk_batch_remove(gk_batch *b, gk_solver *s);
+int gk_batch_size(const gk_batch *b);
+int gk_batch_step(gk_batch *b, int n);        /* n steps, waits for them */
+int gk_batch_step_async(gk_batch *b, int n);  /* n steps, returns at once */
+int gk_batch_sync(gk_batch *b);
+int gk_batch_set_graphs(gk_batch *b, int on); /* default on */
+int gk_batch_enable_timing(gk_batch *b, int on);
+int gk_batch_kernel_times(gk_batch *b, const char **names, double *ms, int *n);
+
 /* Measurement helper (no counterpart in the reference): a plain device copy
    of `bytes` bytes (rounded down to 16-byte units), `reps` times after one
    warm-up; *ms receives the average duration of one copy kernel.  Used to

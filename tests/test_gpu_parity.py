@@ -3,7 +3,8 @@
 The oracle is pinned bit-for-bit to the compiled reference (test_oracle.py);
 here the HIP path must match it to relative L1 <= 1e-10 per conserved
 variable and face-B component after the stated number of steps, with
-max |div B| at FP64 roundoff.
+max |div B| at FP64 roundoff.  The FP64 device path keeps the reference's
+operation order, so it is also required to be bitwise identical.
 """
 import numpy as np
 import pytest
@@ -41,6 +42,7 @@ def test_device_matches_oracle(kind, seed, ni, nj, nk, steps):
         for k, v in rel.items():
             assert np.isfinite(v) and v <= TOL, (k, v)
         assert sc["nonfinite"] == 0.0
+        assert nbad == 0
         assert sc["divB"] < 1e-10
         assert abs(sc["time"] - O.scalar("time_sim")) <= 1e-12 * O.scalar("time_sim")
         assert abs(sc["dt"] - O.scalar("dt")) <= 1e-10 * O.scalar("dt")
