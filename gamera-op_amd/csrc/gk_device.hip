@@ -1314,7 +1314,7 @@ struct gk_solver {
 static int dev_alloc_bytes(gk_solver *s, void **p, size_t nbytes) {
   if (hipMalloc(p, nbytes) != hipSuccess) return GK_ERR_NOMEM;
   s->bytes += nbytes;
-  if (hipMemset(*p, 0, nbytes) != hipSuccess) return GK_ERR_DEVICE;
+  if (hipMemsetAsync(*p, 0, nbytes, s->stream) != hipSuccess) return GK_ERR_DEVICE;
   return GK_OK;
 }
 
@@ -1349,6 +1349,12 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
 
   gk_solver *s = new gk_solver();
+  // every upload of the set-up runs on the solver's own stream, so creating
+  // a solver does not wait for work other solvers or batches have in flight
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return GK_ERR_DEVICE;
+  }
   s->ni = d->ni;
   s->nj = d->nj;
   s->nk = d->nk;
@@ -1415,10 +1421,12 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   };
   if (!cls_ok) return fail(GK_ERR_ARG);
   if ((rc = dev_alloc(s, &s->tables, h.size())) != GK_OK) return fail(rc);
-  if (hipMemcpy(s->tables, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(s->tables, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess)
     return fail(GK_ERR_DEVICE);
   if (hipMalloc((void **)&s->itables, hi.size() * sizeof(int)) != hipSuccess) return fail(GK_ERR_NOMEM);
-  if (hipMemcpy(s->itables, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(s->itables, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess)
     return fail(GK_ERR_DEVICE);
   for (int q = 0; q < GK_G_COUNT; q++)
     for (int a = 0; a < 3; a++) s->geo.g[q][a] = s->tables + off_g[q][a];
@@ -1430,13 +1438,14 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     s->geo.cls[a][1] = s->itables + off_c[a][1];
     s->geo.recwf[a] = nullptr;
   }
+  std::vector<float> hf;  // kept until the final stream synchronisation
   if (s->precision == 1) {  // FP32 copies of the reconstruction tables
-    std::vector<float> hf(h.size());
+    hf.resize(h.size());
     for (size_t q = 0; q < h.size(); q++) hf[q] = (float)h[q];
     if ((rc = dev_alloc_bytes(s, (void **)&s->tables_f, hf.size() * sizeof(float))) != GK_OK)
       return fail(rc);
-    if (hipMemcpy(s->tables_f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice) !=
-        hipSuccess)
+    if (hipMemcpyAsync(s->tables_f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice,
+                       s->stream) != hipSuccess)
       return fail(GK_ERR_DEVICE);
     for (int a = 0; a < 3; a++) s->geo.recwf[a] = s->tables_f + off_w[a];
   }
@@ -1455,11 +1464,9 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if ((rc = dev_alloc_bytes(s, &s->E, 3 * fs * s->esz)) != GK_OK) return fail(rc);
   if (hipMalloc((void **)&s->sc, sizeof(Scal)) != hipSuccess) return fail(GK_ERR_NOMEM);
   Scal z{};
-  if (hipMemcpy(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream) != hipSuccess)
     return fail(GK_ERR_DEVICE);
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(GK_ERR_DEVICE);
-  if (hipDeviceSynchronize() != hipSuccess) return fail(GK_ERR_DEVICE);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
   s->chunk = 32;
   s->cur = 0;
   s->b_m1 = 0;
@@ -2141,6 +2148,7 @@ static void batch_free_tables(gk_batch *b) {
 }
 
 static int batch_build(gk_batch *b) {
+  HIPCHK(hipStreamSynchronize(b->stream));  // nothing in flight uses the old tables
   batch_free_tables(b);
   std::vector<char> hargs[2][6][SK_COUNT];
   std::vector<WorkItem> hwork[2][SK_COUNT];

@@ -72,6 +72,10 @@ def _load():
     _op.gk_get_tuning.restype = ctypes.c_int
     _op.gk_get_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int]
     vp = ctypes.c_void_p
+    h.gkh_main.restype = ctypes.c_int
+    h.gkh_main.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    h.gkh_reseed.restype = ctypes.c_int
+    h.gkh_reseed.argtypes = [ctypes.c_uint64]
     h.gkh_device_create.restype = ctypes.c_int
     h.gkh_device_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
     for name, args in [("gk_autotune", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -104,6 +108,52 @@ def exported_symbols():
     return ["gk_create", "gk_destroy", "gk_set_state", "gk_prologue", "gk_step",
             "gk_get_state", "gk_get_scalars", "gk_set_time", "gk_enable_timing",
             "gk_kernel_times", "gk_device_bytes"]
+
+
+def run_main(config_file, nt):
+    """The reference main() sequence on one GPU in the current directory:
+    config file, restart from this rank's last dump if there is one, problem
+    set-up otherwise, then nt steps with HDF5 dumps at the reference cadence.
+    The host arrays stay readable through HostArrays until finalize()."""
+    h, _ = _load()
+    rc = h.gkh_main(config_file.encode(), nt)
+    if rc != 0:
+        raise GkError(f"gkh_main failed ({rc})")
+    return HostArrays()
+
+
+def finalize():
+    h, _ = _load()
+    h.finalize_solver()
+
+
+class HostArrays:
+    """Views of the host facade's current arrays (reference layout)."""
+
+    def __init__(self):
+        self.h, self.op = _load()
+        sz = (ctypes.c_int * 6)()
+        self.h.gkh_sizes(sz)
+        self.n = (sz[0], sz[1], sz[2])
+        self.shape = (sz[3], sz[4], sz[5])
+
+    def _view(self, which, f):
+        p = self.h.gkh_ptr(which, f)
+        if not p:
+            raise GkError(f"array {which}/{f} is not stored on the host")
+        return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def gas(self, f):
+        return self._view(0, f)
+
+    def gem(self, f):
+        return self._view(1, f)
+
+    def scalar(self, name):
+        return ctypes.c_double.in_dll(self.h, name).value
+
+    def int(self, name):
+        return ctypes.c_int.in_dll(self.h, name).value
 
 
 def copy_calibrate(nbytes, reps=20):
@@ -146,6 +196,13 @@ class Problem:
         if not p:
             raise GkError(f"array {which}/{f} is not stored on the host")
         return np.ctypeslib.as_array(p, shape=self.shape)
+
+    def reseed(self, seed):
+        """Same grid, new seeded initial state (skips the geometry set-up)."""
+        rc = self.h.gkh_reseed(seed)
+        if rc != 0:
+            raise GkError(f"gkh_reseed failed ({rc})")
+        self.seed = seed
 
     def gas(self, f):
         return self._view(0, f)
@@ -268,6 +325,9 @@ class Solver:
         B = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in bf])
         _check(self.op.gk_get_state(self.handle, P, B, None, None), "gk_get_state")
         return prim, bf
+
+    def device_bytes(self):
+        return int(self.op.gk_device_bytes(self.handle))
 
     def autotune(self, trial_steps=3):
         best = ctypes.c_int(0)

@@ -191,10 +191,10 @@ static int verify_config(void) {
   return 0;
 }
 
-/* derived normalisation units (src/config.c:165-171) */
+/* derived normalisation units (src/config.c:165-171); the mu0 member is left
+   at zero as the reference leaves it (restart files compare the struct) */
 static void derive_norms(void) {
   double mu0 = 4.0 * PI * 1e-7;
-  norm_config.mu0 = mu0;
   norm_config.Time_Norm = norm_config.x_Norm / norm_config.u_Norm;
   norm_config.p_Norm = norm_config.rho_Norm * norm_config.u_Norm * norm_config.u_Norm;
   norm_config.B_Norm = sqrt(mu0 * norm_config.rho_Norm) * norm_config.u_Norm;

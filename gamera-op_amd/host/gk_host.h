@@ -5,6 +5,7 @@
 #ifndef GK_HOST_H
 #define GK_HOST_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "gamera_op.h"
@@ -52,5 +53,14 @@ typedef struct {
 int gk_tables_build(gk_tables *t);
 void gk_tables_free(gk_tables *t);
 int gk_fill_desc(gk_grid_desc *d, const gk_tables *t, int precision);
+
+/* HDF5 dumps, restart and grid files (built when HDF5 is available) */
+extern int gk_io_enabled;
+int dump_data_hdf5(const char *base);
+int dump_extra_data(const char *base);
+int read_data_hdf5(const char *filename);
+int gk_read_restart_file(const char *base);
+int gk_read_grid_file(void);
+void get_out_filename(const char *base, char *filename, size_t size, int seq, const char *ext);
 
 #endif

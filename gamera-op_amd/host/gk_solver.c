@@ -9,8 +9,11 @@
  * lives in device memory between calls; the host arrays are refreshed after
  * solve() returns and whenever a dump is due.
  */
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -21,6 +24,11 @@
 #include "log.h"
 #include "problem.h"
 #include "solver.h"
+
+/* problem selection of the seeded plugin (problems/gk_seeded.c) */
+extern int gk_problem_kind;
+extern uint64_t gk_problem_seed;
+extern int gk_problem_seed_set;
 
 /* optional problem hook: rewrite x1/x2/x3 before the metrics are computed */
 int gk_problem_custom_grid(void) __attribute__((weak));
@@ -66,24 +74,78 @@ int allocate_solver() {
   return 0;
 }
 
+/* dumps and restart lookup: on for the reference's main() sequence, off for
+   the programmatic set-up (gkh_setup), which never touches the disk */
+int gk_io_enabled = 1;
+
+/* The HDF5 layer lives in libgamera_h5.so next to this library (it links the
+   HDF5 runtime); it is loaded on first use.  Without it the facade runs
+   without dumps, restarts or grid files and says so once. */
+typedef struct {
+  int (*dump)(const char *);
+  int (*extra)(const char *);
+  int (*restart)(const char *);
+  int (*grid)(void);
+} gk_h5_api;
+
+static const gk_h5_api *gk_h5(void) {
+  static int tried = 0, ok = 0;
+  static gk_h5_api api;
+  if (!tried) {
+    tried = 1;
+    Dl_info info;
+    if (dladdr((void *)&gk_h5, &info) && info.dli_fname) {
+      char path[4096];
+      snprintf(path, sizeof path, "%s", info.dli_fname);
+      char *slash = strrchr(path, '/');
+      if (slash)
+        snprintf(slash + 1, sizeof path - (size_t)(slash + 1 - path), "%s", "libgamera_h5.so");
+      else
+        snprintf(path, sizeof path, "%s", "libgamera_h5.so");
+      void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+      if (h) {
+        api.dump = (int (*)(const char *))dlsym(h, "dump_data_hdf5");
+        api.extra = (int (*)(const char *))dlsym(h, "dump_extra_data");
+        api.restart = (int (*)(const char *))dlsym(h, "gk_read_restart_file");
+        api.grid = (int (*)(void))dlsym(h, "gk_read_grid_file");
+        ok = api.dump && api.extra && api.restart && api.grid;
+      }
+    }
+    if (!ok) log_warn("HDF5 layer not available: no dumps, restarts or grid files");
+  }
+  return ok ? &api : NULL;
+}
+
 int set_domain_decomposition() {
   gk_default_decomposition();
+  /* a grid file in the working directory overrides the uniform grid */
+  if (gk_io_enabled && gk_h5()) {
+    int st = gk_h5()->grid();
+    if (st < 0) {
+      log_error("Error reading grid.h5");
+      return -1;
+    }
+  }
   return 0;
 }
 
 int initialize_solver(const char *base_name) {
-  (void)base_name;
   log_info("Allocating solver");
   if (allocate_solver() != 0) {
     log_error("host allocation failed");
     return -1;
   }
-  /* restart files are handled by the HDF5 layer when it is present */
   read_restart = false;
   time_sim = 0.0;
   time_sim_start = 0.0;
   if (set_domain_decomposition() != 0) return -1;
-  if (gk_problem_custom_grid) gk_problem_custom_grid();
+  int st = 1;
+  /* restart from this rank's last dump (src/solver_init.c:352-376) */
+  if (gk_io_enabled && gk_h5()) st = gk_h5()->restart(base_name);
+  if (st < 0) return st;
+  read_restart = st == 0;
+  if (read_restart) time_sim_start = time_sim;
+  if (!read_restart && gk_problem_custom_grid) gk_problem_custom_grid();
   if (set_geometry_arrays() != 0) return -1;
   log_info("Solver initialized");
   return 0;
@@ -200,12 +262,19 @@ int gkh_device_open(int precision) {
   rc = gk_create(&d, &g_dev);
   if (rc != GK_OK) return rc;
   g_precision = precision;
-  const double *prim[6], *bf[3];
-  for (int m = 0; m < 6; m++) prim[m] = &gas[0][m][0][0][0];
-  for (int m = 0; m < 3; m++) bf[m] = &gem[m][0][0][0];
-  rc = gk_set_state(g_dev, prim, bf, NULL, NULL);
+  const double *prim[6], *bf[3], *pp[6], *bp[3];
+  for (int m = 0; m < 6; m++) {
+    prim[m] = &gas[0][m][0][0][0];
+    pp[m] = &gas[0][gas_rho_p + m][0][0][0];
+  }
+  for (int m = 0; m < 3; m++) {
+    bf[m] = &gem[m][0][0][0];
+    bp[m] = &gem[mag_bi_p + m][0][0][0];
+  }
+  /* a restart carries both time levels and dt0 (src/solver_solve.c:55-92) */
+  rc = read_restart ? gk_set_state(g_dev, prim, bf, pp, bp) : gk_set_state(g_dev, prim, bf, NULL, NULL);
   if (rc != GK_OK) return rc;
-  rc = gk_prologue(g_dev, 0.0);
+  rc = gk_prologue(g_dev, read_restart ? dt0 : 0.0);
   if (rc != GK_OK) return rc;
   rc = gk_set_time(g_dev, time_sim);
   g_dev_fresh = 0;
@@ -243,6 +312,38 @@ int gkh_device_create(int precision, gk_solver **out) {
   return GK_OK;
 }
 
+/* cell-centred B of the previous time level (what the reference's
+   X_p := X copy leaves in mags_p), for dumps */
+static void host_mags_prev(void) {
+  for (int i = isg; i <= ieg; i++)
+    for (int j = jsg; j <= jeg; j++)
+      for (int k = ksg; k <= keg; k++) {
+        gem[mags_b1_p][i][j][k] = gem[mag_bi_p][i][j][k] * (x1[i + 1][j][k] - x1ctr[i][j][k]) / dx1[i][j][k] +
+                                  gem[mag_bi_p][i + 1][j][k] * (x1ctr[i][j][k] - x1[i][j][k]) / dx1[i][j][k];
+        gem[mags_b2_p][i][j][k] = gem[mag_bj_p][i][j][k] * (x2[i][j + 1][k] - x2ctr[i][j][k]) / dx2[i][j][k] +
+                                  gem[mag_bj_p][i][j + 1][k] * (x2ctr[i][j][k] - x2[i][j][k]) / dx2[i][j][k];
+        gem[mags_b3_p][i][j][k] = gem[mag_bk_p][i][j][k] * (x3[i][j][k + 1] - x3ctr[i][j][k]) / dx3[i][j][k] +
+                                  gem[mag_bk_p][i][j][k + 1] * (x3ctr[i][j][k] - x3[i][j][k]) / dx3[i][j][k];
+      }
+}
+
+int gkh_device_fetch(void);
+
+/* device state into the host arrays; without history only the current
+   time level (the initial dump happens before X_p := X in the reference) */
+static int device_fetch(int history) {
+  if (!g_dev) return GK_ERR_STATE;
+  double *prim[6], *bf[3];
+  for (int m = 0; m < 6; m++) prim[m] = &gas[0][m][0][0][0];
+  for (int m = 0; m < 3; m++) bf[m] = &gem[m][0][0][0];
+  int rc;
+  if (history) return gkh_device_fetch();
+  rc = gk_get_state(g_dev, prim, bf, NULL, NULL);
+  if (rc != GK_OK) return rc;
+  get_derived_variables();
+  return GK_OK;
+}
+
 int gkh_device_fetch(void) {
   if (!g_dev) return GK_ERR_STATE;
   double *prim[6], *bf[3], *pp[6], *bp[3];
@@ -264,6 +365,7 @@ int gkh_device_fetch(void) {
   time_sim = sc[2];
   divB_max = sc[3];
   get_derived_variables();
+  host_mags_prev();
   return GK_OK;
 }
 
@@ -277,7 +379,19 @@ int solve(int nt) {
     }
   }
   time_sim_start = time_sim;
-  const int per_step = time_stop < 1e300 && time_stop > 0.0;
+  const gk_h5_api *io = gk_io_enabled ? gk_h5() : NULL;
+  const int dumps = io != NULL;
+  double last_dump_time = time_sim;
+  if (dumps && !read_restart) {
+    /* initial state after ghost fill and BCs (src/solver_solve.c:44-53) */
+    hdf_seq_num = -1;
+    log_seq_num = 0;
+    if (device_fetch(0) != GK_OK) return -1;
+    io->dump(BASE_DAT_NAME);
+    io->extra(BASE_EXT_NAME);
+    last_dump_time = 0;
+  }
+  const int per_step = dumps || (time_stop < 1e300 && time_stop > 0.0);
   if (!per_step) {
     rc = gk_step(g_dev, nt);
     if (rc != GK_OK) return -1;
@@ -286,34 +400,85 @@ int solve(int nt) {
       rc = gk_step(g_dev, 1);
       if (rc != GK_OK) return -1;
       double sc[6];
-      gk_get_scalars(g_dev, sc);
+      if (gk_get_scalars(g_dev, sc) != GK_OK) return -1;
       if (sc[5] != 0.0) log_error("non-finite state detected at step %d", n);
-      if (time_sim_start + 0.0 < sc[2] && sc[2] >= 0.1 * output_interval * log_seq_num) {
+      time_sim = sc[2];
+      dt = sc[0];
+      divB_max = sc[3];
+      if (dumps && time_sim >= output_interval * (hdf_seq_num + 1)) {
+        if (gkh_device_fetch() != GK_OK) return -1;
+        io->dump(BASE_DAT_NAME);
+        log_info("Output data at time %.15f", time_sim);
+        last_dump_time = time_sim;
+      }
+      if (time_sim >= 0.1 * output_interval * log_seq_num) {
         log_info("Status at step %d: rank 0, Time %.15f, dt %.15f, max_divB for this rank is %.4e", n,
-                 sc[2], sc[0], sc[3]);
+                 time_sim, dt, divB_max);
         log_seq_num++;
       }
-      if (sc[2] > time_stop) {
-        log_info("Step %d, Time %f, dt %f", n, sc[2], sc[0]);
+      if (time_sim > time_stop) {
+        log_info("Step %d, Time %f, dt %f", n, time_sim, dt);
         break;
       }
     }
   }
   rc = gkh_device_fetch();
-  return rc == GK_OK ? 0 : -1;
+  if (rc != GK_OK) return -1;
+  if (dumps && last_dump_time != time_sim) io->dump(BASE_DAT_NAME);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Single-GPU stand-ins for the reference's communicator set-up
+   (src/setup_mpi.h:26-29): one rank owns the whole grid, so the periodic
+   images come from the device ghost fill. */
+int setup_comm_and_types(int rank) {
+  (void)rank;
+  if (config.proc_dims[0] * config.proc_dims[1] * config.proc_dims[2] != 1) {
+    log_error("the device solver runs one rank per grid (proc_dims must be 1 1 1)");
+    return -1;
+  }
+  config.proc_coords[0] = config.proc_coords[1] = config.proc_coords[2] = 0;
+  return 0;
+}
+
+void free_mpi_types(void) {}
+
+/* The reference main() sequence for one rank (src/main.c:30-75), without
+   MPI: config file, logging, solver set-up or restart, weights, problem
+   initial state (fresh start only), background field, then solve(nt) with
+   the reference's dump cadence.  Arrays stay allocated for inspection until
+   finalize_solver(). */
+int gkh_main(const char *config_file, int nt) {
+  if (gas) finalize_solver();
+  gk_io_enabled = 1;
+  gk_problem_kind = -1; /* taken from the environment (GK_PROBLEM, GK_SEED) */
+  gk_problem_seed_set = 0;
+  B10 = B20 = B30 = Field_Default;
+  Source1_B0 = Source2_B0 = Source3_B0 = Field_Default;
+  A1 = A2 = A3 = Field_Default;
+  set_problem_config();
+  if (initialize_config(0, 1, config_file) != 0) return -1;
+  if (setup_comm_and_types(0) != 0) return -1;
+  if (initialize_logging(BASE_LOG_NAME, 0) != 0) return -1;
+  if (initialize_solver(BASE_DAT_NAME) != 0) return -2;
+  getweights();
+  if (!read_restart) problem_init();
+  set_background_field();
+  g_dev_fresh = 1;
+  if (nt > 0 && solve(nt) != 0) return -3;
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */
 /* programmatic set-up used by the Python bindings, tests and the benchmark:
    the reference main() sequence (src/main.c:30-63) without a YAML file */
-extern int gk_problem_kind;
-extern uint64_t gk_problem_seed;
-extern int gk_problem_seed_set;
 
 static gk_tables g_probe;
 void gkh_table_reset(void);
 int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int full_b0) {
   if (gas) finalize_solver();
+  gk_io_enabled = 0;
   gkh_table_reset();
   gk_problem_kind = kind;
   gk_problem_seed = seed;
@@ -344,6 +509,32 @@ int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int fu
 
 /* host array access for the bindings: 0 gas f, 1 gem f, 2 geo f, 3 rec f,
    4.. coordinates x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
+/* New initial state for the problem already set up, with another seed: the
+   grid, geometry and reconstruction weights do not depend on the seed, so
+   only the problem's parameters, initial state and background field are
+   redone.  Equal, array for array, to gkh_setup with the new seed. */
+int gkh_reseed(uint64_t seed) {
+  if (!gas || !gem) return -1;
+  gk_problem_seed = seed;
+  gk_problem_seed_set = 1;
+  B10 = B20 = B30 = Field_Default;
+  Source1_B0 = Source2_B0 = Source3_B0 = Field_Default;
+  A1 = A2 = A3 = Field_Default;
+  time_sim = 0.0;
+  set_problem_config();
+  const size_t n3 = (size_t)config.NI * config.NJ * config.NK;
+  for (int f = 0; f < NF_gas; f++)
+    if (gas_keep[f] && gas[0][f]) memset(&gas[0][f][0][0][0], 0, n3 * sizeof(double));
+  for (int f = 0; f < NF_gem; f++)
+    if (gem_keep[f] && gem[f]) memset(&gem[f][0][0][0], 0, n3 * sizeof(double));
+  problem_init();
+  set_background_field();
+  if (gk_b0.mode < 0) return -3;
+  record_bcs();
+  g_dev_fresh = 1;
+  return 0;
+}
+
 double *gkh_ptr(int which, int f) {
   double ***c3[12] = {x1, x2, x3, x1c, x2c, x3c, x1ctr, x2ctr, x3ctr, dx1, dx2, dx3};
   switch (which) {

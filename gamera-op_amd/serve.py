@@ -1,0 +1,237 @@
+"""Serving loop for many independent MHD solves on one GPU.
+
+Requests (a seeded problem, a grid size, a step count and a precision tag)
+arrive over time.  They are admitted while the device-memory budget and the
+slot count allow, their state stays on the device, and every admitted solve
+advances in the same batched steps (one launch per stage for all of them):
+continuous batching, where requests join and leave the batch between step
+quanta.  The host set-up of newly admitted requests runs while the GPU
+executes the current quantum.  A solve's results do not depend on what else
+is in the batch (batched stepping runs each grid's own arithmetic).
+
+The plain loop (`serve_plain`) is the batching-off baseline: one request at a
+time, set up, stepped to completion and retired before the next starts.
+"""
+import collections
+import hashlib
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import gamera_op as go
+
+CART_SIZES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128)]
+SPH_SIZES = [(64, 32, 64), (96, 48, 96), (128, 64, 128)]
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float       # seconds after the start of the stream
+    kind: int
+    seed: int
+    shape: tuple
+    steps: int
+    precision: int       # 0 FP64, 1 FP32-bulk
+
+    @property
+    def cells(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    cell_updates: int
+    time_sim: float
+    digest: str = ""
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def make_requests(n, rate, seed=0, frac_cart=0.7, steps=(50, 500), cart_sizes=CART_SIZES,
+                  sph_sizes=SPH_SIZES, frac_fp32=0.5):
+    """Seeded Poisson arrivals at `rate` per second with the configured mix."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n)) if rate > 0 else np.zeros(n)
+    out = []
+    for q in range(n):
+        if rng.random() < frac_cart:
+            kind, shape = go.SINUSOID_BLAST, cart_sizes[rng.integers(len(cart_sizes))]
+        else:
+            kind, shape = go.DIPOLE_SHELL, sph_sizes[rng.integers(len(sph_sizes))]
+        out.append(Request(q, float(t[q]), kind, int(rng.integers(1, 2**31)), tuple(shape),
+                           int(rng.integers(steps[0], steps[1] + 1)),
+                           1 if rng.random() < frac_fp32 else 0))
+    return out
+
+
+def device_bytes_estimate(shape, precision):
+    """Device bytes gk_create allocates for a grid (state, history,
+    intermediates; the per-axis tables are negligible)."""
+    N = [n + 9 for n in shape]
+    kp = (N[2] + 15) // 16 * 16
+    fs = N[0] * N[1] * kp
+    esz = 4 if precision == 1 else 8
+    return 8 * fs * (12 + 9 + 9) + esz * fs * (27 + 6 + 3)
+
+
+def state_digest(solver):
+    prim, bf = solver.state()
+    h = hashlib.sha256()
+    for a in prim + bf:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+class _Setup:
+    """Host set-up of one request at a time (the host facade holds one
+    problem); consecutive requests of the same kind and grid reuse the grid,
+    geometry and weights and only redo the seeded initial state."""
+
+    def __init__(self):
+        self.P = None
+        self.key = None
+
+    def solver(self, r, tune):
+        key = (r.kind, r.shape)
+        if self.P is not None and self.key == key:
+            self.P.reseed(r.seed)
+        else:
+            if self.P is not None:
+                self.P.close()
+            self.P = go.Problem(r.kind, r.seed, *r.shape)
+            self.key = key
+        S = go.Solver(self.P, r.precision)
+        if tune:
+            S.autotune(2)
+        return S
+
+    def close(self):
+        if self.P is not None:
+            self.P.close()
+            self.P = None
+
+
+def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, digests=False,
+          realtime=True, log=None):
+    """Run the request stream through the continuous-batching loop.
+    Returns (results, wall seconds)."""
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    setup = _Setup()
+    B = go.Batch()
+    queue = collections.deque()
+    active = []  # [request, solver, steps_left, admitted_time, bytes]
+    pending = []
+    results = []
+    used = 0
+    nxt = 0
+    in_flight = 0
+    t0 = time.perf_counter()
+
+    def now():
+        return time.perf_counter() - t0
+
+    try:
+        while nxt < len(reqs) or queue or active or pending:
+            tn = now()
+            while nxt < len(reqs) and (reqs[nxt].arrival <= tn or not realtime):
+                queue.append(reqs[nxt])
+                nxt += 1
+            # admit (host set-up overlaps the quantum in flight); one request
+            # per pass while the GPU is busy, as many as fit when it is idle
+            while queue and len(active) + len(pending) < max_slots:
+                r = queue[0]
+                need = device_bytes_estimate(r.shape, r.precision)
+                if used + need > budget_bytes and (active or pending):
+                    break
+                queue.popleft()
+                S = setup.solver(r, tune)
+                used += S.device_bytes()
+                pending.append([r, S, r.steps, now(), S.device_bytes()])
+                if in_flight:
+                    break
+            if in_flight:
+                B.sync()
+                done = []
+                for a in active:
+                    a[2] -= in_flight
+                    if a[2] == 0:
+                        done.append(a)
+                in_flight = 0
+                for a in done:
+                    r, S = a[0], a[1]
+                    B.remove(S)
+                    sc = S.scalars()
+                    res = Result(r.rid, r.arrival, a[3], now(), r.steps, r.cells * r.steps,
+                                 sc["time"])
+                    if digests:
+                        res.digest = state_digest(S)
+                    results.append(res)
+                    S.close()
+                    used -= a[4]
+                    active.remove(a)
+                    if log:
+                        log(f"done rid={r.rid} latency={res.latency:.3f}s active={len(active)}")
+            for p in pending:
+                B.add(p[1])
+                active.append(p)
+            pending = []
+            if active:
+                n = min(quantum, min(a[2] for a in active))
+                B.step_async(n)
+                in_flight = n
+            elif nxt < len(reqs) and not queue:
+                time.sleep(max(0.0, min(reqs[nxt].arrival - now(), 0.05)))
+        wall = now()
+    finally:
+        B.close()
+        for a in active + pending:
+            a[1].close()
+        setup.close()
+    return results, wall
+
+
+def serve_plain(requests, tune=False, digests=False, realtime=True):
+    """Batching off: one request at a time, in arrival order."""
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    setup = _Setup()
+    results = []
+    t0 = time.perf_counter()
+    try:
+        for r in reqs:
+            if realtime:
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            adm = time.perf_counter() - t0
+            S = setup.solver(r, tune)
+            S.step(r.steps)
+            sc = S.scalars()
+            res = Result(r.rid, r.arrival, adm, time.perf_counter() - t0, r.steps,
+                         r.cells * r.steps, sc["time"])
+            if digests:
+                res.digest = state_digest(S)
+            results.append(res)
+            S.close()
+    finally:
+        setup.close()
+    return results, time.perf_counter() - t0
+
+
+def summarize(results, wall):
+    lat = np.array([r.latency for r in results])
+    cu = sum(r.cell_updates for r in results)
+    return {"requests": len(results), "wall_s": wall, "cell_updates": cu,
+            "cell_updates_per_s": cu / wall if wall > 0 else 0.0,
+            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "latency_max_s": float(lat.max()) if len(lat) else None}

@@ -33,11 +33,13 @@ def ref_available(spherical=False):
     return os.path.exists(exe)
 
 
-def run_reference(kind, seed, ni, nj, nk, nt, geometry=False, threads=8):
-    """Run the reference for nt steps; returns the record dict."""
+def run_reference(kind, seed, ni, nj, nk, nt, geometry=False, threads=8, workdir=None):
+    """Run the reference for nt steps; returns the record dict.  With workdir
+    the run happens there and its files (HDF5 dumps, logs) are kept."""
     spherical = kind == 2
     exe = os.path.join(REF_DIR, "ref_driver_sph" if spherical else "ref_driver_cart")
-    with tempfile.TemporaryDirectory(prefix="gkref") as d:
+    with tempfile.TemporaryDirectory(prefix="gkref") as tmp:
+        d = workdir or tmp
         with open(os.path.join(d, "c.yaml"), "w") as f:
             f.write(YAML.format(ni=ni, nj=nj, nk=nk, nt=nt))
         env = dict(os.environ, GK_PROBLEM=str(kind), GK_SEED=str(seed),
