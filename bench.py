@@ -127,6 +127,84 @@ def run_gpu(args):
     return el, kt, sc, cbytes / (cms * 1e-3) / 1e9, fp32
 
 
+ENSEMBLE_SIZES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 32)]
+
+
+def run_ensemble(args):
+    """Ensemble of independent Cartesian periodic grids (sizes drawn by seed
+    from ENSEMBLE_SIZES; the sinusoid generator with its own seed, half of
+    them with the over-pressure sphere), FP32-bulk: all grids advanced in
+    batched steps (one launch per stage for the whole ensemble, HIP graphs,
+    autotuned march lengths) against the same grids stepped one after the
+    other."""
+    sys.path.insert(0, PKG)
+    import numpy as np
+    import gamera_op as go
+
+    rng = np.random.default_rng(1000 + args.seed)
+    specs = []
+    for q in range(args.ens_instances):
+        shape = ENSEMBLE_SIZES[rng.integers(len(ENSEMBLE_SIZES))]
+        kind = go.SINUSOID_SPHERE if rng.random() < 0.5 else go.SINUSOID
+        specs.append((kind, int(rng.integers(1, 2**31)), shape))
+    log(f"bench: ensemble of {len(specs)} grids, set-up")
+    solvers = []
+    cells = 0
+    for kind, sd, shape in sorted(specs, key=lambda t: (t[0], t[2])):
+        P = go.Problem(kind, sd, *shape)
+        S = go.Solver(P, 1)
+        P.close()
+        S.autotune(2)
+        solvers.append(S)
+        cells += shape[0] * shape[1] * shape[2]
+    steps = args.ens_steps
+    B = go.Batch()
+    for S in solvers:
+        B.add(S)
+    B.step(6)  # warm-up, graph capture
+    log(f"bench: ensemble batched, {steps} steps")
+    t0 = time.perf_counter()
+    B.step(steps)
+    el_b = time.perf_counter() - t0
+    B.close()
+    log(f"bench: ensemble one grid at a time, {steps} steps")
+    t0 = time.perf_counter()
+    for S in solvers:
+        S.step(steps)
+    el_u = time.perf_counter() - t0
+    for S in solvers:
+        S.close()
+    return {"instances": len(specs), "cells": cells, "steps": steps, "precision": "fp32-bulk",
+            "batched_cell_updates_per_s": cells * steps / el_b,
+            "one_at_a_time_cell_updates_per_s": cells * steps / el_u,
+            "speedup": el_u / el_b, "batched_s": el_b, "one_at_a_time_s": el_u,
+            "sizes": "drawn from " + str(ENSEMBLE_SIZES)}
+
+
+def run_serving(args):
+    """Served stream (scaled): seeded Poisson arrivals, 70% Cartesian
+    (sinusoid + sphere, 32^3..128^3) / 30% spherical shell (dipole + blob),
+    steps U(50, 500), half FP32-bulk, admitted under a fixed device-memory
+    budget; continuous batching against the batching-off plain loop on the
+    same requests."""
+    sys.path.insert(0, PKG)
+    import serve
+
+    reqs = serve.make_requests(args.serve_requests, args.serve_rate, seed=args.seed)
+    log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, budget "
+        f"{args.serve_budget_gib} GiB")
+    res, wall = serve.serve(reqs, budget_bytes=args.serve_budget_gib << 30)
+    srv = serve.summarize(res, wall)
+    log(f"bench: served {srv}")
+    log("bench: plain loop on the same requests")
+    res_p, wall_p = serve.serve_plain(reqs)
+    pl = serve.summarize(res_p, wall_p)
+    log(f"bench: plain {pl}")
+    return {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
+            "budget_GiB": args.serve_budget_gib, "served": srv, "plain_loop": pl,
+            "throughput_ratio": srv["cell_updates_per_s"] / pl["cell_updates_per_s"]}
+
+
 def run_cpu(args):
     """CPU restatement on a bounded sample of the same workload."""
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -166,6 +244,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0, help="faces per thread in the i/j sweeps")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-instances", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=300)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs a single-GPU domain "
@@ -189,6 +274,8 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fp:
             traffic = json.load(fp).get("bytes_per_launch")
+    ens = None if args.no_ensemble else run_ensemble(args)
+    srv = None if args.no_serve else run_serving(args)
     cpu = None if args.no_cpu else run_cpu(args)
     step_ms_kernels = sum(kt.values())
     out = {
@@ -218,6 +305,8 @@ def main():
         "copy_kernel_GBs": copy_gbs,
         "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
         "fp32_bulk": fp32,
+        "ensemble": ens,
+        "serving": srv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -65,7 +65,7 @@ def make_requests(n, rate, seed=0, frac_cart=0.7, steps=(50, 500), cart_sizes=CA
     out = []
     for q in range(n):
         if rng.random() < frac_cart:
-            kind, shape = go.SINUSOID_BLAST, cart_sizes[rng.integers(len(cart_sizes))]
+            kind, shape = go.SINUSOID_SPHERE, cart_sizes[rng.integers(len(cart_sizes))]
         else:
             kind, shape = go.DIPOLE_SHELL, sph_sizes[rng.integers(len(sph_sizes))]
         out.append(Request(q, float(t[q]), kind, int(rng.integers(1, 2**31)), tuple(shape),
