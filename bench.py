@@ -181,7 +181,7 @@ def run_ensemble(args):
             "sizes": "drawn from " + str(ENSEMBLE_SIZES)}
 
 
-def run_serving(args):
+def run_serving(args, preparer):
     """Served stream (scaled): seeded Poisson arrivals, 70% Cartesian
     (sinusoid + sphere, 32^3..128^3) / 30% spherical shell (dipole + blob),
     steps U(50, 500), half FP32-bulk, admitted under a fixed device-memory
@@ -193,7 +193,7 @@ def run_serving(args):
     reqs = serve.make_requests(args.serve_requests, args.serve_rate, seed=args.seed)
     log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, budget "
         f"{args.serve_budget_gib} GiB")
-    res, wall = serve.serve(reqs, budget_bytes=args.serve_budget_gib << 30)
+    res, wall = serve.serve(reqs, budget_bytes=args.serve_budget_gib << 30, preparer=preparer)
     srv = serve.summarize(res, wall)
     log(f"bench: served {srv}")
     log("bench: plain loop on the same requests")
@@ -201,7 +201,9 @@ def run_serving(args):
     pl = serve.summarize(res_p, wall_p)
     log(f"bench: plain {pl}")
     return {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
-            "budget_GiB": args.serve_budget_gib, "served": srv, "plain_loop": pl,
+            "budget_GiB": args.serve_budget_gib,
+            "preparation_workers": preparer.workers if preparer else 0,
+            "served": srv, "plain_loop": pl,
             "throughput_ratio": srv["cell_updates_per_s"] / pl["cell_updates_per_s"]}
 
 
@@ -251,6 +253,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--serve-workers", type=int, default=0,
+                    help="request-preparation processes (default: host cores - 1, at most 16)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs a single-GPU domain "
@@ -258,6 +262,12 @@ def main():
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
+    preparer = None
+    if not args.no_serve:
+        # request-preparation processes start before this process touches the GPU
+        sys.path.insert(0, PKG)
+        import serve
+        preparer = serve.Preparer(args.serve_workers or None)
 
     el, kt, sc, copy_gbs, fp32 = run_gpu(args)
     n = args.n
@@ -275,7 +285,9 @@ def main():
         with open(tfile) as fp:
             traffic = json.load(fp).get("bytes_per_launch")
     ens = None if args.no_ensemble else run_ensemble(args)
-    srv = None if args.no_serve else run_serving(args)
+    srv = None if args.no_serve else run_serving(args, preparer)
+    if preparer is not None:
+        preparer.close()
     cpu = None if args.no_cpu else run_cpu(args)
     step_ms_kernels = sum(kt.values())
     out = {

@@ -334,6 +334,34 @@ __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R b
   }
 }
 
+// XCD-aware block order.  Workgroups of a launch are dealt to the 8 XCDs in
+// turn and every XCD has its own L2; a 1-D launch is therefore decoded so that
+// each XCD receives a contiguous range of the logical blocks (logical order
+// z fastest when ZF, x fastest otherwise).  Neighbouring blocks, which share
+// stencil halos, chunk prologues and faces, then meet in the same L2.
+constexpr int NXCD = 8;
+
+template <bool ZF>
+__device__ __forceinline__ bool xcd_block(int gx, int gy, int gz, int &bx, int &by, int &bz) {
+  const int total = gx * gy * gz;
+  const int per = (total + NXCD - 1) / NXCD;
+  const int b = blockIdx.x;
+  const int L = (b % NXCD) * per + b / NXCD;
+  if (L >= total) return false;
+  if (ZF) {
+    bz = L % gz;
+    const int t = L / gz;
+    bx = t % gx;
+    by = t / gx;
+  } else {
+    bx = L % gx;
+    const int t = L / gx;
+    by = t % gy;
+    bz = t / gy;
+  }
+  return true;
+}
+
 // work-list entry of a batched launch: argument set and block coordinates
 struct WorkItem {
   int arg, bx, by, bz;
@@ -344,9 +372,11 @@ struct SweepArgs {
   Grid g;
   Geo geo;
   B0c b0;
-  const double *X, *Xp;  // 6 fields
-  const double *M, *Mp;  // 3 cell-B fields at n and n-1
-  const double *B;       // face B at n (face-normal component for the stress)
+  // state read by the stencil: of type R (FP64 state, or its FP32 copy in
+  // the FP32-bulk form)
+  const void *X, *Xp;    // 6 fields
+  const void *M, *Mp;    // 3 cell-B fields at n and n-1
+  const void *B;         // face B at n (face-normal component for the stress)
   void *F;               // 9 fields of R: 6 hydro + 3 magnetic stress
   void *V;               // 2 fields of R: face-averaged velocity components
   const Scal *sc;
@@ -355,13 +385,15 @@ struct SweepArgs {
   int chunk;             // faces per thread in the marching sweeps
 };
 
-// Direct sweep: each thread reads its eight stencil cells (used for the
-// velocity-only strips of the j sweep).
+// Direct sweep for the velocity-only strips of the j sweep (k in the four
+// ghost layers on either side): each thread reads its eight stencil cells;
+// a wave covers the 4 strip cells of 16 consecutive j rows.
 template <int D, typename R>
 __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
-  const int k = a.lo[2] + bx * blockDim.x + threadIdx.x;
-  const int j = a.lo[1] + by * 4 + threadIdx.y;
+  (void)bx;
+  const int k = a.lo[2] + (threadIdx.x & (NGH - 1));
+  const int j = a.lo[1] + by * 64 + threadIdx.y * 16 + (threadIdx.x >> 2);
   const int i = a.lo[0] + bz;
   if (k > a.hi[2] || j > a.hi[1]) return;
   const Grid &g = a.g;
@@ -382,7 +414,7 @@ __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, c
 #pragma unroll
   for (int m = 0; m < 6; m++) {
     if (!full && m != va && m != vb) continue;
-    recon(a.X + m * g.fs, a.Xp + m * g.fs, c, sd, cc, T, L[m], Rs[m]);
+    recon((const R *)a.X + m * g.fs, (const R *)a.Xp + m * g.fs, c, sd, cc, T, L[m], Rs[m]);
   }
   V[c] = R(0.5) * (L[va] + Rs[va]);
   V[g.fs + c] = R(0.5) * (L[vb] + Rs[vb]);
@@ -390,26 +422,33 @@ __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, c
 
   R bl[3], br[3];
 #pragma unroll
-  for (int m = 0; m < 3; m++) recon(a.M + m * g.fs, a.Mp + m * g.fs, c, sd, cc, T, bl[m], br[m]);
+  for (int m = 0; m < 3; m++)
+    recon((const R *)a.M + m * g.fs, (const R *)a.Mp + m * g.fs, c, sd, cc, T, bl[m], br[m]);
 
   R Fo[9];
-  face_flux<D, R>(L, Rs, bl, br, (R)a.B[D * g.fs + c], a.b0, g.CA, Fo);
+  face_flux<D, R>(L, Rs, bl, br, ((const R *)a.B)[D * g.fs + c], a.b0, g.CA, Fo);
 #pragma unroll
   for (int m = 0; m < 9; m++) F[m * g.fs + c] = Fo[m];
 }
 
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
-  k_sweep_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
+  k_sweep_body<D, R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_sweep_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_sweep_body<D, R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -427,9 +466,9 @@ struct Window {
   R x[NF][6];  // current values at cells f-2 .. f+3
 };
 
-template <int NF, typename R>
-__device__ __forceinline__ void march_load(Window<NF, R> &w, const double *const P[NF],
-                                           const double *const Q[NF], long o, R cc, int slot) {
+template <int NF, typename R, typename S>
+__device__ __forceinline__ void march_load(Window<NF, R> &w, const S *const P[NF],
+                                           const S *const Q[NF], long o, R cc, int slot) {
 #pragma unroll
   for (int m = 0; m < NF; m++) {
     const R x = (R)P[m][o];
@@ -486,23 +525,24 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
   constexpr int vb = D == 0 ? 2 : 3;
   const long fs = g.fs;
   R *F = (R *)a.F, *V = (R *)a.V;
+  const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
   if (r >= a.flo[RA] && r <= a.fhi[RA]) {
-    const double *P[9], *Q[9];
+    const R *P[9], *Q[9];
 #pragma unroll
     for (int m = 0; m < 6; m++) {
-      P[m] = a.X + m * fs;
-      Q[m] = a.Xp + m * fs;
+      P[m] = X + m * fs;
+      Q[m] = Xp + m * fs;
     }
 #pragma unroll
     for (int m = 0; m < 3; m++) {
-      P[6 + m] = a.M + m * fs;
-      Q[6 + m] = a.Mp + m * fs;
+      P[6 + m] = (const R *)a.M + m * fs;
+      Q[6 + m] = (const R *)a.Mp + m * fs;
     }
     Window<9, R> w;
 #pragma unroll
-    for (int p = 0; p < 7; p++) march_load<9, R>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int p = 0; p < 7; p++) march_load<9, R, R>(w, P, Q, c + (p - 4) * sd, cc, p);
     for (int f = f0; f < f1; f++) {
-      march_load<9, R>(w, P, Q, c + 3 * sd, cc, 7);
+      march_load<9, R, R>(w, P, Q, c + 3 * sd, cc, 7);
       R L[6], Rs[6], bl[3], br[3];
 #pragma unroll
       for (int m = 0; m < 6; m++) march_lr(w, m, T, L[m], Rs[m]);
@@ -511,7 +551,7 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
       V[c] = R(0.5) * (L[va] + Rs[va]);
       V[fs + c] = R(0.5) * (L[vb] + Rs[vb]);
       R Fo[9];
-      face_flux<D, R>(L, Rs, bl, br, (R)a.B[D * fs + c], a.b0, g.CA, Fo);
+      face_flux<D, R>(L, Rs, bl, br, ((const R *)a.B)[D * fs + c], a.b0, g.CA, Fo);
 #pragma unroll
       for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
       march_shift(w);
@@ -519,13 +559,13 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
       T += tstep;
     }
   } else {
-    const double *P[2] = {a.X + va * fs, a.X + vb * fs};
-    const double *Q[2] = {a.Xp + va * fs, a.Xp + vb * fs};
+    const R *P[2] = {X + va * fs, X + vb * fs};
+    const R *Q[2] = {Xp + va * fs, Xp + vb * fs};
     Window<2, R> w;
 #pragma unroll
-    for (int p = 0; p < 7; p++) march_load<2, R>(w, P, Q, c + (p - 4) * sd, cc, p);
+    for (int p = 0; p < 7; p++) march_load<2, R, R>(w, P, Q, c + (p - 4) * sd, cc, p);
     for (int f = f0; f < f1; f++) {
-      march_load<2, R>(w, P, Q, c + 3 * sd, cc, 7);
+      march_load<2, R, R>(w, P, Q, c + 3 * sd, cc, 7);
       R l, rr;
       march_lr(w, 0, T, l, rr);
       V[c] = R(0.5) * (l + rr);
@@ -539,17 +579,23 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
 }
 
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_march(SweepArgs a) {
-  k_march_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_march(SweepArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_march_body<D, R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_march_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_march_body<D, R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -561,6 +607,7 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *args, cons
 // face-averaged velocities (v3, v1).
 constexpr int KROWS = 4;
 constexpr int KSPAN = 72;
+constexpr int CELL_CHUNK = 1;  // cells each thread of the cell update walks along i
 
 template <typename R>
 __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, const int by,
@@ -581,7 +628,7 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
   const long rowb = i * g.st[0] + min(j0 + ty, g.N[1] - 1) * g.st[1];
   const long oa = rowb + min(k0 - 4 + tx, g.N[2] - 1);
   const long ob = rowb + min(k0 + 60 + tx, g.N[2] - 1);
-  auto stage = [&](int m, const double *P, const double *Q) {
+  auto stage = [&](int m, const R *P, const R *Q) {
     const R xa = (R)P[oa], pa = (R)Q[oa];
     sh[m][ty][tx] = xa + cc * (xa - pa);
     sx[m][ty][tx] = xa;
@@ -591,14 +638,16 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
       sx[m][ty][64 + tx] = xb;
     }
   };
+  const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
+  const R *M = (const R *)a.M, *Mp = (const R *)a.Mp;
   if (full) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) stage(m, a.X + m * fs, a.Xp + m * fs);
+    for (int m = 0; m < 6; m++) stage(m, X + m * fs, Xp + m * fs);
 #pragma unroll
-    for (int m = 0; m < 3; m++) stage(6 + m, a.M + m * fs, a.Mp + m * fs);
+    for (int m = 0; m < 3; m++) stage(6 + m, M + m * fs, Mp + m * fs);
   } else {
-    stage(3, a.X + 3 * fs, a.Xp + 3 * fs);
-    stage(1, a.X + 1 * fs, a.Xp + 1 * fs);
+    stage(3, X + 3 * fs, Xp + 3 * fs);
+    stage(1, X + 1 * fs, Xp + 1 * fs);
   }
   __syncthreads();
   const int k = k0 + threadIdx.x;
@@ -634,23 +683,29 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
   V[c] = R(0.5) * (L[3] + Rs[3]);
   V[fs + c] = R(0.5) * (L[1] + Rs[1]);
   R Fo[9];
-  face_flux<2, R>(L, Rs, bl, br, (R)a.B[2 * fs + c], a.b0, g.CA, Fo);
+  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], a.b0, g.CA, Fo);
 #pragma unroll
   for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_sweepk(SweepArgs a) {
-  k_sweepk_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_sweepk(SweepArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
+  k_sweepk_body<R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <typename R>
-__global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_sweepk_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_sweepk_body<R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -697,8 +752,8 @@ struct EmfArgs {
   Grid g;
   Geo geo;
   B0c b0;
-  const double *X, *Xp;
-  const double *B, *Bp;
+  const void *X, *Xp;  // of type R (FP64 state or its FP32 copy)
+  const void *B, *Bp;
   const void *V;  // 2 fields of R: face-averaged velocity (from the feeding sweep)
   void *E;        // 1 field of R
   const Scal *sc;
@@ -765,9 +820,9 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   const R cc = (R)a.sc->c;
   const R *V0 = (const R *)a.V, *V1 = (const R *)a.V + fs;
   R *E = (R *)a.E;
-  const double *B1 = a.B + F1 * fs, *B1p = a.Bp + F1 * fs;
-  const double *B2 = a.B + F2 * fs, *B2p = a.Bp + F2 * fs;
-  const double *X = a.X, *Xp = a.Xp;
+  const R *B1 = (const R *)a.B + F1 * fs, *B1p = (const R *)a.Bp + F1 * fs;
+  const R *B2 = (const R *)a.B + F2 * fs, *B2p = (const R *)a.Bp + F2 * fs;
+  const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
   auto rh = [&](long o) {
     R x = (R)X[o];
     return x + cc * (x - (R)Xp[o]);
@@ -776,8 +831,8 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   const long tstep = (long)a.geo.ncls[MA][0] * a.geo.ncls[MA][1] * GK_RECW_STRIDE;
   Window1<R> wv0, wv1;
   Window<1, R> wb;
-  const double *Bw = VW ? B1 : B2, *Bwp = VW ? B1p : B2p;
-  const double *Pw[1] = {Bw}, *Qw[1] = {Bwp};
+  const R *Bw = VW ? B1 : B2, *Bwp = VW ? B1p : B2p;
+  const R *Pw[1] = {Bw}, *Qw[1] = {Bwp};
 #pragma unroll
   for (int p = 0; p < 7; p++) {
     const long o = c + (p - 4) * sm;
@@ -785,7 +840,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       wv0.h[p] = V0[o];
       wv1.h[p] = V1[o];
     }
-    march_load<1, R>(wb, Pw, Qw, o, cc, p);
+    march_load<1, R, R>(wb, Pw, Qw, o, cc, p);
   }
   // densities at march position f0 - 1 (offsets -1 and 0 along O)
   R u0 = rh(c - sm - so), u1 = rh(c - sm);
@@ -795,7 +850,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       wv0.h[7] = V0[o];
       wv1.h[7] = V1[o];
     }
-    march_load<1, R>(wb, Pw, Qw, o, cc, 7);
+    march_load<1, R, R>(wb, Pw, Qw, o, cc, 7);
     idx[MA] = f;
     R va, vb, b1l, b1r, b2l, b2r, l, rr;
     if (VW) {
@@ -835,17 +890,23 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
 }
 
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_emf_march(EmfArgs a) {
-  k_emf_march_body<D, R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_emf_march_body<D, R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_emf_march_batch(const EmfArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_emf_march_batch(const EmfArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_emf_march_body<D, R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_emf_march_body<D, R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -856,6 +917,7 @@ struct CtArgs {
   Geo geo;
   const double *B;   // B^n
   double *Bn;        // B^{n+1}
+  float *Bn32;       // FP32 copy of B^{n+1} (FP32-bulk form), else null
   const void *E;     // 3 edge fields of R
   const Scal *sc;
 };
@@ -880,32 +942,44 @@ __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const i
   if (aj && ak) {  // B_i on faces i = is..ie+1
     double d = ((double)Ek[c + SJ] * geo3(G, GK_G_EDGE_K, i, j + 1, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k)) -
                ((double)Ej[c + 1] * geo3(G, GK_G_EDGE_J, i, j, k + 1) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k));
-    a.Bn[c] = a.B[c] - dt * d / geo3(G, GK_G_FACE_I, i, j, k);
+    const double b = a.B[c] - dt * d / geo3(G, GK_G_FACE_I, i, j, k);
+    a.Bn[c] = b;
+    if (a.Bn32) a.Bn32[c] = (float)b;
   }
   if (ai && ak) {  // B_j on faces j = js..je+1
     double d = ((double)Ei[c + 1] * geo3(G, GK_G_EDGE_I, i, j, k + 1) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k)) -
                ((double)Ek[c + SI] * geo3(G, GK_G_EDGE_K, i + 1, j, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k));
-    a.Bn[g.fs + c] = a.B[g.fs + c] - dt * d / geo3(G, GK_G_FACE_J, i, j, k);
+    const double b = a.B[g.fs + c] - dt * d / geo3(G, GK_G_FACE_J, i, j, k);
+    a.Bn[g.fs + c] = b;
+    if (a.Bn32) a.Bn32[g.fs + c] = (float)b;
   }
   if (ai && aj) {  // B_k on faces k = ks..ke+1
     double d = ((double)Ej[c + SI] * geo3(G, GK_G_EDGE_J, i + 1, j, k) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k)) -
                ((double)Ei[c + SJ] * geo3(G, GK_G_EDGE_I, i, j + 1, k) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k));
-    a.Bn[2 * g.fs + c] = a.B[2 * g.fs + c] - dt * d / geo3(G, GK_G_FACE_K, i, j, k);
+    const double b = a.B[2 * g.fs + c] - dt * d / geo3(G, GK_G_FACE_K, i, j, k);
+    a.Bn[2 * g.fs + c] = b;
+    if (a.Bn32) a.Bn32[2 * g.fs + c] = (float)b;
   }
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_ct(CtArgs a) {
-  k_ct_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_ct(CtArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_ct_body<R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <typename R>
-__global__ __launch_bounds__(256) void k_ct_batch(const CtArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_ct_batch(const CtArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_ct_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_ct_body<R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -918,6 +992,7 @@ struct CellArgs {
   int spherical;
   const double *X, *Xp;    // X^n, X^{n-1}
   double *Xn;              // X^{n+1} (aliases Xp: each cell reads before it writes)
+  float *Xn32;             // FP32 copy of X^{n+1} (FP32-bulk form), else null
   const double *M, *Mp;    // cell B at n, n-1
   const double *Bn;        // face B at n+1
   const void *F[3];        // 9 fields of R per direction
@@ -933,13 +1008,18 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
                                        const int bz) {
   const Grid &g = a.g;
   const Geo &G = a.geo;
-  const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
+  const int k = g.s[2] + bx * 64 + threadIdx.x;
   const int j = g.s[1] + by * 4 + threadIdx.y;
-  const int i = g.s[0] + bz;
+  // each thread walks CELL_CHUNK cells along i, so the i-face values a cell
+  // shares with the next (fluxes, face B) are re-read from cache, not HBM
+  const int i0 = g.s[0] + bz * CELL_CHUNK;
+  const int i1 = min(i0 + CELL_CHUNK, g.e[0] + 1);
   const bool valid = k <= g.e[2] && j <= g.e[1];
   double dtc = INFINITY, divb = 0.0;
   bool bad = false;
-  if (valid) {
+  if (valid)
+#pragma unroll 1
+  for (int i = i0; i < i1; i++) {
     const long c = i * g.st[0] + j * g.st[1] + k;
     const int idx[3] = {i, j, k};
     const double dt = a.sc->dt;
@@ -1070,36 +1150,50 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
       a.Xn[m * g.fs + c] = out[m];
       bad |= !isfinite(out[m]);
     }
+    if (a.Xn32) {
+#pragma unroll
+      for (int m = 0; m < 6; m++) a.Xn32[m * g.fs + c] = (float)out[m];
+    }
     // next step's CFL dt from the new state (magtot = cell B + B0)
-    dtc = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0], bnew[1] + a.b0.cell[1],
-                 bnew[2] + a.b0.cell[2], geo3(G, GK_G_DXI, i, j, k), geo3(G, GK_G_DXJ, i, j, k),
-                 geo3(G, GK_G_DXK, i, j, k), CA);
-    if (!(dtc == dtc)) dtc = INFINITY;  // fmin in the reference ignores NaN
+    double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0],
+                          bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2],
+                          geo3(G, GK_G_DXI, i, j, k), geo3(G, GK_G_DXJ, i, j, k),
+                          geo3(G, GK_G_DXK, i, j, k), CA);
+    if (dcell == dcell) dtc = fmin(dtc, dcell);  // fmin in the reference ignores NaN
     // div B of the new face field (with the B0 face normals)
     const double *Bf = a.Bn;
     const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
     double db = ((Bf[c + g.st[0]] + f0[0]) * A1[0] - (Bf[c] + f0[0]) * A0[0]) / V +
                 ((Bf[g.fs + c + g.st[1]] + f1[1]) * A1[1] - (Bf[g.fs + c] + f1[1]) * A0[1]) / V +
                 ((Bf[2 * g.fs + c + 1] + f2[2]) * A1[2] - (Bf[2 * g.fs + c] + f2[2]) * A0[2]) / V;
-    divb = fabs(db);
-    if (!(divb == divb)) bad = true;
+    const double adb = fabs(db);
+    if (adb == adb)
+      divb = fmax(divb, adb);
+    else
+      bad = true;
   }
   block_reduce(dtc, divb, a.sc);
   if (bad) atomicOr(&a.sc->nanflag, 1u);
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_cell(CellArgs a) {
-  k_cell_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_cell(CellArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_cell_body<R>(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <typename R>
-__global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_cell_body<R>(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_cell_body<R>(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -1154,8 +1248,10 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
 // point another thread writes.
 struct GhostArgs {
   Grid g;
-  double *X;  // 6 cell fields
-  double *B;  // 3 face fields
+  double *X;    // 6 cell fields
+  double *B;    // 3 face fields
+  float *X32;   // FP32 copies (FP32-bulk form), else null
+  float *B32;
   int bc[2][6];
 };
 
@@ -1191,6 +1287,10 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
     if (src != c) {
 #pragma unroll
       for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = a.X[m * g.fs + src];
+      if (a.X32) {
+#pragma unroll
+        for (int m = 0; m < 6; m++) a.X32[m * g.fs + c] = a.X32[m * g.fs + src];
+      }
     }
   }
 #pragma unroll
@@ -1200,20 +1300,29 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
     for (int d = 0; d < 3; d++)
       m[d] = gmap(x[d], g.s[d], g.e[d], a.bc[1][2 * d], a.bc[1][2 * d + 1], d == f ? 1 : 0);
     long src = m[0] * g.st[0] + m[1] * g.st[1] + m[2];
-    if (src != c) a.B[f * g.fs + c] = a.B[f * g.fs + src];
+    if (src != c) {
+      a.B[f * g.fs + c] = a.B[f * g.fs + src];
+      if (a.B32) a.B32[f * g.fs + c] = a.B32[f * g.fs + src];
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
-  k_ghost_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void k_ghost(GhostArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_ghost_body(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
-__global__ __launch_bounds__(256) void k_ghost_batch(const GhostArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_ghost_batch(const GhostArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_ghost_body(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_ghost_body(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -1225,6 +1334,7 @@ struct MagsArgs {
   Geo geo;
   const double *B;
   double *M;
+  float *M32;  // FP32 copy (FP32-bulk form), else null
 };
 
 __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
@@ -1237,20 +1347,37 @@ __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, con
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int idx[3] = {i, j, k};
 #pragma unroll
-  for (int m = 0; m < 3; m++)
-    a.M[m * g.fs + c] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
+  for (int m = 0; m < 3; m++) {
+    const double b = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
+    a.M[m * g.fs + c] = b;
+    if (a.M32) a.M32[m * g.fs + c] = (float)b;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_mags(MagsArgs a) {
-  k_mags_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+// FP32 copy of n FP64 values (FP32-bulk form set-up)
+__global__ __launch_bounds__(256) void k_to32(const double *__restrict__ src,
+                                              float *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    dst[q] = (float)src[q];
+}
+
+__global__ __launch_bounds__(256) void k_mags(MagsArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_mags_body(a, bx, by, bz);
 }
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
-__global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *args, const WorkItem *work,
-                                                    int nwork) {
+__global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__ args,
+                                                    const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const WorkItem it = work[w];
-    k_mags_body(args[it.arg], it.bx, it.by, it.bz);
+    // uniform per block: keep the argument set in scalar registers
+    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+    k_mags_body(args[arg], bx, by, bz);
     __syncthreads();
   }
 }
@@ -1279,6 +1406,9 @@ struct gk_solver {
   double *tables;      // device copy of all per-axis tables
   int *itables;        // class maps
   double *X[2];        // 6 fields each: X[cur] = X^n, X[1-cur] = X^{n-1}
+  float *X32[2];       // FP32 copies of X, B, M read by the FP32-bulk stencils
+  float *B32[3];
+  float *M32[3];
   double *B[3];        // face B rotation: B[b_m1], B[b_0] = B^n, B[b_n]
   double *M[3];        // cell-centred B, rotating with B
   void *F[3];          // 9 face-flux fields per direction (type by precision)
@@ -1457,6 +1587,17 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     if ((rc = dev_alloc(s, &s->B[b], 3 * fs)) != GK_OK) return fail(rc);
     if ((rc = dev_alloc(s, &s->M[b], 3 * fs)) != GK_OK) return fail(rc);
   }
+  if (s->precision == 1) {
+    for (int b = 0; b < 2; b++)
+      if ((rc = dev_alloc_bytes(s, (void **)&s->X32[b], 6 * fs * sizeof(float))) != GK_OK)
+        return fail(rc);
+    for (int b = 0; b < 3; b++) {
+      if ((rc = dev_alloc_bytes(s, (void **)&s->B32[b], 3 * fs * sizeof(float))) != GK_OK)
+        return fail(rc);
+      if ((rc = dev_alloc_bytes(s, (void **)&s->M32[b], 3 * fs * sizeof(float))) != GK_OK)
+        return fail(rc);
+    }
+  }
   for (int d2 = 0; d2 < 3; d2++) {
     if ((rc = dev_alloc_bytes(s, &s->F[d2], 9 * fs * s->esz)) != GK_OK) return fail(rc);
     if ((rc = dev_alloc_bytes(s, &s->V[d2], 2 * fs * s->esz)) != GK_OK) return fail(rc);
@@ -1483,10 +1624,15 @@ extern "C" int gk_destroy(gk_solver *s) {
   (void)hipFree(s->tables);
   (void)hipFree(s->tables_f);
   (void)hipFree(s->itables);
-  for (int b = 0; b < 2; b++) (void)hipFree(s->X[b]);
+  for (int b = 0; b < 2; b++) {
+    (void)hipFree(s->X[b]);
+    (void)hipFree(s->X32[b]);
+  }
   for (int b = 0; b < 3; b++) {
     (void)hipFree(s->B[b]);
     (void)hipFree(s->M[b]);
+    (void)hipFree(s->B32[b]);
+    (void)hipFree(s->M32[b]);
   }
   for (int d = 0; d < 3; d++) {
     (void)hipFree(s->F[d]);
@@ -1525,9 +1671,12 @@ static void launch_ghost(gk_solver *s, double *X, double *B) {
   a.g = s->g;
   a.X = X;
   a.B = B;
+  a.X32 = nullptr;
+  a.B32 = nullptr;
   memcpy(a.bc, s->bc, sizeof(a.bc));
-  hipLaunchKernelGGL(k_ghost, box_grid(s->g.N[2], s->g.N[1], s->g.N[0]), dim3(64, 4), 0,
-                     s->stream, a);
+  const dim3 gr = box_grid(s->g.N[2], s->g.N[1], s->g.N[0]);
+  const int tot = gr.x * gr.y * gr.z;
+  k_ghost<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
 }
 
 static void launch_mags(gk_solver *s, const double *B, double *M) {
@@ -1536,8 +1685,22 @@ static void launch_mags(gk_solver *s, const double *B, double *M) {
   a.geo = s->geo;
   a.B = B;
   a.M = M;
-  hipLaunchKernelGGL(k_mags, box_grid(s->g.N[2] - 1, s->g.N[1] - 1, s->g.N[0] - 1), dim3(64, 4),
-                     0, s->stream, a);
+  a.M32 = nullptr;
+  const dim3 gr = box_grid(s->g.N[2] - 1, s->g.N[1] - 1, s->g.N[0] - 1);
+  const int tot = gr.x * gr.y * gr.z;
+  k_mags<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
+}
+
+// FP32 copies of both time levels (FP32-bulk form), after the state is set
+static void refresh_copies32(gk_solver *s) {
+  if (s->precision != 1) return;
+  const size_t fs = s->fs;
+  const dim3 grid(2048), blk(256);
+  for (int b = 0; b < 2; b++) hipLaunchKernelGGL(k_to32, grid, blk, 0, s->stream, s->X[b], s->X32[b], 6 * fs);
+  for (int b = 0; b < 3; b++) {
+    hipLaunchKernelGGL(k_to32, grid, blk, 0, s->stream, s->B[b], s->B32[b], 3 * fs);
+    hipLaunchKernelGGL(k_to32, grid, blk, 0, s->stream, s->M[b], s->M32[b], 3 * fs);
+  }
 }
 
 extern "C" int gk_set_state(gk_solver *s, const double *const prim[6], const double *const bface[3],
@@ -1585,6 +1748,7 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
     launch_ghost(s, s->X[1 - s->cur], s->B[s->b_m1]);
     launch_mags(s, s->B[s->b_m1], s->M[s->b_m1]);
   }
+  refresh_copies32(s);
   Scal cur;
   HIPCHK(hipMemcpyAsync(&cur, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1730,6 +1894,14 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
   double *Bn = s->B[r.b_n], *Mn = s->M[r.b_n];
   double *Xn = s->X[1 - r.cur];
   const int ch = s->chunk;
+  // the stencil kernels read the state in their compute type
+  const bool f32 = s->precision == 1;
+  const void *Xs = f32 ? (const void *)s->X32[r.cur] : (const void *)X;
+  const void *Xps = f32 ? (const void *)s->X32[1 - r.cur] : (const void *)Xp;
+  const void *Ms = f32 ? (const void *)s->M32[r.b_0] : (const void *)M;
+  const void *Mps = f32 ? (const void *)s->M32[r.b_m1] : (const void *)Mm1;
+  const void *Bs = f32 ? (const void *)s->B32[r.b_0] : (const void *)B;
+  const void *Bps = f32 ? (const void *)s->B32[r.b_m1] : (const void *)Bm1;
   Launch L;
   memset(&L, 0, sizeof L);
   L.kid = SK_BEGIN;
@@ -1744,11 +1916,11 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
-    a.X = X;
-    a.Xp = Xp;
-    a.M = M;
-    a.Mp = Mm1;
-    a.B = B;
+    a.X = Xs;
+    a.Xp = Xps;
+    a.M = Ms;
+    a.Mp = Mps;
+    a.B = Bs;
     a.F = s->F[D];
     a.V = s->V[D];
     a.sc = s->sc;
@@ -1786,9 +1958,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
         SweepArgs v = a;
         v.lo[2] = side == 0 ? 0 : Ee[2] + 1;
         v.hi[2] = side == 0 ? S[2] - 1 : g.N[2] - 2;
-        L.kid = SK_VSTRIP_J;
-        L.grid = dim3((v.hi[2] - v.lo[2] + 64) / 64, (v.hi[1] - v.lo[1] + 4) / 4,
-                      v.hi[0] - v.lo[0] + 1);
+        L.kid = SK_VSTRIP_J;  // strip width is NGH = 4 on both sides
+        L.grid = dim3(1, (v.hi[1] - v.lo[1] + 64) / 64, v.hi[0] - v.lo[0] + 1);
         L.u.sw = v;
         out.push_back(L);
       }
@@ -1807,10 +1978,10 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
-    a.X = X;
-    a.Xp = Xp;
-    a.B = B;
-    a.Bp = Bm1;
+    a.X = Xs;
+    a.Xp = Xps;
+    a.B = Bs;
+    a.Bp = Bps;
     // E_i uses the j-sweep velocities, E_j the k-sweep, E_k the i-sweep
     a.V = s->V[D == 0 ? 1 : (D == 1 ? 2 : 0)];
     a.E = (char *)s->E + (size_t)D * s->fs * s->esz;
@@ -1834,6 +2005,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.geo = s->geo;
     a.B = B;
     a.Bn = Bn;
+    a.Bn32 = f32 ? s->B32[r.b_n] : nullptr;
     a.E = s->E;
     a.sc = s->sc;
     L.kid = SK_CT;
@@ -1851,13 +2023,14 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.X = X;
     a.Xp = Xp;
     a.Xn = Xn;
+    a.Xn32 = f32 ? s->X32[1 - r.cur] : nullptr;
     a.M = M;
     a.Mp = Mm1;
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
     L.kid = SK_CELL;
-    L.grid = box_grid(s->nk, s->nj, s->ni);
+    L.grid = dim3((s->nk + 63) / 64, (s->nj + 3) / 4, (s->ni + CELL_CHUNK - 1) / CELL_CHUNK);
     L.u.ce = a;
     out.push_back(L);
   }
@@ -1867,6 +2040,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.X = Xn;
     a.B = Bn;
+    a.X32 = f32 ? s->X32[1 - r.cur] : nullptr;
+    a.B32 = f32 ? s->B32[r.b_n] : nullptr;
     memcpy(a.bc, s->bc, sizeof(a.bc));
     L.kid = SK_GHOST;
     L.grid = box_grid(g.N[2], g.N[1], g.N[0]);
@@ -1880,6 +2055,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.geo = s->geo;
     a.B = Bn;
     a.M = Mn;
+    a.M32 = f32 ? s->M32[r.b_n] : nullptr;
     L.kid = SK_MAGS;
     L.grid = box_grid(g.N[2] - 1, g.N[1] - 1, g.N[0] - 1);
     L.u.mg = a;
@@ -1890,19 +2066,22 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
 template <typename R>
 static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 blk(64, 4);
+  const int gx = L.grid.x, gy = L.grid.y, gz = L.grid.z;
+  const int total = gx * gy * gz;
+  const dim3 g1((total + NXCD - 1) / NXCD * NXCD);  // decoded by xcd_block
   switch (L.kid) {
     case SK_BEGIN: k_begin<<<1, 1, 0, st>>>(L.u.sc); break;
-    case SK_SWEEP_I: k_march<0, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
-    case SK_SWEEP_J: k_march<1, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
-    case SK_VSTRIP_J: k_sweep<1, R><<<L.grid, blk, 0, st>>>(L.u.sw); break;
-    case SK_SWEEP_K: k_sweepk<R><<<L.grid, dim3(64, KROWS), 0, st>>>(L.u.sw); break;
-    case SK_EMF_I: k_emf_march<0, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
-    case SK_EMF_J: k_emf_march<1, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
-    case SK_EMF_K: k_emf_march<2, R><<<L.grid, blk, 0, st>>>(L.u.em); break;
-    case SK_CT: k_ct<R><<<L.grid, blk, 0, st>>>(L.u.ct); break;
-    case SK_CELL: k_cell<R><<<L.grid, blk, 0, st>>>(L.u.ce); break;
-    case SK_GHOST: k_ghost<<<L.grid, blk, 0, st>>>(L.u.gh); break;
-    case SK_MAGS: k_mags<<<L.grid, blk, 0, st>>>(L.u.mg); break;
+    case SK_SWEEP_I: k_march<0, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64, KROWS), 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_EMF_I: k_emf_march<0, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_J: k_emf_march<1, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_K: k_emf_march<2, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_CT: k_ct<R><<<g1, blk, 0, st>>>(L.u.ct, gx, gy, gz); break;
+    case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
+    case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
+    case SK_MAGS: k_mags<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
   }
 }
 
@@ -2436,6 +2615,8 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
                      : hipMemcpyAsync(p, s->M[q], 3 * fs * 8, kd, s->stream));
     HIPCHK(restore ? hipMemcpyAsync(s->sc, p, sizeof(Scal), kd, s->stream)
                    : hipMemcpyAsync(p, s->sc, sizeof(Scal), kd, s->stream));
+    // the FP32 copies always equal the rounded FP64 state: rebuild them
+    if (restore) refresh_copies32(s);
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur = r0.cur;
     s->b_m1 = r0.b_m1;

@@ -72,6 +72,13 @@ def _load():
     _op.gk_get_tuning.restype = ctypes.c_int
     _op.gk_get_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int]
     vp = ctypes.c_void_p
+    h.gkh_desc_blob.restype = ctypes.c_long
+    h.gkh_desc_blob.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    h.gkh_solver_from_blob.restype = ctypes.c_int
+    h.gkh_solver_from_blob.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p)]
     h.gkh_main.restype = ctypes.c_int
     h.gkh_main.argtypes = [ctypes.c_char_p, ctypes.c_int]
     h.gkh_reseed.restype = ctypes.c_int
@@ -210,6 +217,18 @@ class Problem:
     def gem(self, f):
         return self._view(1, f)
 
+    def desc_blob(self):
+        """The problem's grid description (geometry, weights, BCs, B0) as
+        bytes; with the initial state it is all a solver needs."""
+        n = self.h.gkh_desc_blob(None, 0)
+        if n < 0:
+            raise GkError(f"gkh_desc_blob failed ({n})")
+        buf = ctypes.create_string_buffer(n)
+        m = self.h.gkh_desc_blob(buf, n)
+        if m != n:
+            raise GkError(f"gkh_desc_blob failed ({m})")
+        return buf.raw
+
     def geo(self, f):
         return self._view(2, f)
 
@@ -307,6 +326,27 @@ class Solver:
         self.handle = p
         self.n, self.shape = problem.n, problem.shape
         self.precision = precision
+
+    @classmethod
+    def from_parts(cls, blob, prim, bface, precision=0):
+        """Solver from a descriptor blob (Problem.desc_blob) and an initial
+        state: 6 primitive and 3 face-B arrays in the reference layout."""
+        self = cls.__new__(cls)
+        self.h, self.op = _load()
+        shape = prim[0].shape
+        for a in list(prim) + list(bface):
+            if a.shape != shape or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise GkError("state arrays must be C-contiguous float64 of one shape")
+        P = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in prim])
+        B = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in bface])
+        p = ctypes.c_void_p()
+        _check(self.h.gkh_solver_from_blob(blob, len(blob), precision, P, B, ctypes.byref(p)),
+               "gkh_solver_from_blob")
+        self.handle = p
+        self.shape = shape
+        self.n = tuple(x - 9 for x in shape)
+        self.precision = precision
+        return self
 
     def step(self, n):
         _check(self.op.gk_step(self.handle, n), "gk_step")

@@ -283,6 +283,176 @@ int gkh_device_open(int precision) {
 
 gk_solver *gkh_device(void) { return g_dev; }
 
+/* ------------------------------------------------------------------------ */
+/* Serialised grid description.  A process that set up a problem writes its
+   descriptor (scalars plus every per-axis table) into one flat buffer; any
+   process can rebuild a gk_grid_desc pointing into that buffer and create a
+   solver from it with the initial state it was handed.  Used to prepare
+   requests in worker processes (the host facade holds one problem per
+   process). */
+typedef struct {
+  uint32_t magic, version;
+  int ni, nj, nk, coord, bc[2][6], b0_mode, N[3], ncls[3][2];
+  double CA, b0_cell[3], b0_face[3][8], b0_edge[3][3], b0_lorentz[3];
+} gk_blob_head;
+#define GK_BLOB_MAGIC 0x6b426c6fu
+
+static size_t blob_layout(const gk_blob_head *h, size_t off[GK_G_COUNT + 3 + 1 + 1]) {
+  size_t n = sizeof(gk_blob_head);
+  n = (n + 7) / 8 * 8;
+  size_t q = 0;
+  for (int g = 0; g < GK_G_COUNT; g++) {
+    off[q++] = n;
+    n += sizeof(double) * (size_t)(h->N[0] + h->N[1] + h->N[2]);
+  }
+  for (int a = 0; a < 3; a++) {
+    off[q++] = n;
+    n += sizeof(double) * 3 * (size_t)h->N[a];
+  }
+  off[q++] = n; /* recw of the three axes */
+  for (int a = 0; a < 3; a++)
+    n += sizeof(double) * (size_t)h->N[a] * h->ncls[a][0] * h->ncls[a][1] * GK_RECW_STRIDE;
+  off[q++] = n; /* class maps */
+  for (int a = 0; a < 3; a++) {
+    const int tb = a == 0 ? 1 : 0, tc = a == 2 ? 1 : 2;
+    n += sizeof(int) * (size_t)(h->N[tb] + h->N[tc]);
+  }
+  return n;
+}
+
+/* Write the current problem's descriptor into buf (cap bytes); returns the
+   size needed (call with buf = NULL to ask), or a negative gk_status. */
+long gkh_desc_blob(void *buf, size_t cap) {
+  record_bcs();
+  gk_tables t;
+  if (gk_tables_build(&t) != 0) return GK_ERR_ARG;
+  gk_grid_desc d;
+  int rc = gk_fill_desc(&d, &t, 0);
+  if (rc != GK_OK) {
+    gk_tables_free(&t);
+    return rc;
+  }
+  gk_blob_head h;
+  memset(&h, 0, sizeof h);
+  h.magic = GK_BLOB_MAGIC;
+  h.version = 1;
+  h.ni = d.ni;
+  h.nj = d.nj;
+  h.nk = d.nk;
+  h.coord = d.coord;
+  memcpy(h.bc, d.bc, sizeof h.bc);
+  h.b0_mode = d.b0_mode;
+  h.N[0] = config.NI;
+  h.N[1] = config.NJ;
+  h.N[2] = config.NK;
+  memcpy(h.ncls, d.ncls, sizeof h.ncls);
+  h.CA = d.CA;
+  memcpy(h.b0_cell, d.b0_cell, sizeof h.b0_cell);
+  memcpy(h.b0_face, d.b0_face, sizeof h.b0_face);
+  memcpy(h.b0_edge, d.b0_edge, sizeof h.b0_edge);
+  memcpy(h.b0_lorentz, d.b0_lorentz, sizeof h.b0_lorentz);
+  size_t off[GK_G_COUNT + 5];
+  const size_t need = blob_layout(&h, off);
+  if (buf && cap >= need) {
+    char *b = (char *)buf;
+    memset(b, 0, need);
+    memcpy(b, &h, sizeof h);
+    size_t q = 0;
+    for (int g = 0; g < GK_G_COUNT; g++, q++) {
+      double *dst = (double *)(b + off[q]);
+      for (int a = 0; a < 3; a++) {
+        memcpy(dst, d.geom[g][a], sizeof(double) * h.N[a]);
+        dst += h.N[a];
+      }
+    }
+    for (int a = 0; a < 3; a++, q++)
+      for (int c = 0; c < 3; c++)
+        memcpy(b + off[q] + sizeof(double) * (size_t)c * h.N[a], d.cell_coef[a][c],
+               sizeof(double) * h.N[a]);
+    char *w = b + off[q++];
+    for (int a = 0; a < 3; a++) {
+      const size_t nw = (size_t)h.N[a] * h.ncls[a][0] * h.ncls[a][1] * GK_RECW_STRIDE;
+      memcpy(w, d.recw[a], sizeof(double) * nw);
+      w += sizeof(double) * nw;
+    }
+    char *cm = b + off[q++];
+    for (int a = 0; a < 3; a++) {
+      const int tb = a == 0 ? 1 : 0, tc = a == 2 ? 1 : 2;
+      memcpy(cm, d.cls[a][0], sizeof(int) * h.N[tb]);
+      cm += sizeof(int) * h.N[tb];
+      memcpy(cm, d.cls[a][1], sizeof(int) * h.N[tc]);
+      cm += sizeof(int) * h.N[tc];
+    }
+  }
+  gk_tables_free(&t);
+  return (long)need;
+}
+
+/* Solver from a descriptor blob and an initial state (reference layout,
+   current time level only); prologue done, time set to 0. */
+int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double *const prim[6],
+                         const double *const bface[3], gk_solver **out) {
+  if (!blob || !out || n < sizeof(gk_blob_head)) return GK_ERR_ARG;
+  *out = NULL;
+  gk_blob_head h;
+  memcpy(&h, blob, sizeof h);
+  if (h.magic != GK_BLOB_MAGIC || h.version != 1) return GK_ERR_ARG;
+  size_t off[GK_G_COUNT + 5];
+  if (blob_layout(&h, off) > n) return GK_ERR_ARG;
+  const char *b = (const char *)blob;
+  gk_grid_desc d;
+  memset(&d, 0, sizeof d);
+  d.ni = h.ni;
+  d.nj = h.nj;
+  d.nk = h.nk;
+  d.coord = h.coord;
+  memcpy(d.bc, h.bc, sizeof d.bc);
+  d.precision = precision;
+  d.CA = h.CA;
+  size_t q = 0;
+  for (int g = 0; g < GK_G_COUNT; g++, q++) {
+    const double *src = (const double *)(b + off[q]);
+    for (int a = 0; a < 3; a++) {
+      d.geom[g][a] = src;
+      src += h.N[a];
+    }
+  }
+  for (int a = 0; a < 3; a++, q++)
+    for (int c = 0; c < 3; c++)
+      d.cell_coef[a][c] = (const double *)(b + off[q] + sizeof(double) * (size_t)c * h.N[a]);
+  const char *w = b + off[q++];
+  for (int a = 0; a < 3; a++) {
+    d.recw[a] = (const double *)w;
+    w += sizeof(double) * (size_t)h.N[a] * h.ncls[a][0] * h.ncls[a][1] * GK_RECW_STRIDE;
+  }
+  const char *cm = b + off[q++];
+  for (int a = 0; a < 3; a++) {
+    const int tb = a == 0 ? 1 : 0, tc = a == 2 ? 1 : 2;
+    d.cls[a][0] = (const int *)cm;
+    cm += sizeof(int) * h.N[tb];
+    d.cls[a][1] = (const int *)cm;
+    cm += sizeof(int) * h.N[tc];
+  }
+  memcpy(d.ncls, h.ncls, sizeof d.ncls);
+  d.b0_mode = h.b0_mode;
+  memcpy(d.b0_cell, h.b0_cell, sizeof d.b0_cell);
+  memcpy(d.b0_face, h.b0_face, sizeof d.b0_face);
+  memcpy(d.b0_edge, h.b0_edge, sizeof d.b0_edge);
+  memcpy(d.b0_lorentz, h.b0_lorentz, sizeof d.b0_lorentz);
+  gk_solver *s = NULL;
+  int rc = gk_create(&d, &s);
+  if (rc != GK_OK) return rc;
+  rc = gk_set_state(s, prim, bface, NULL, NULL);
+  if (rc == GK_OK) rc = gk_prologue(s, 0.0);
+  if (rc == GK_OK) rc = gk_set_time(s, 0.0);
+  if (rc != GK_OK) {
+    gk_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return GK_OK;
+}
+
 /* A standalone device solver for the problem currently held by the host
    facade (geometry, weights, B0, initial state): the caller owns it and may
    set up the next problem right away (batched ensembles, serving). */

@@ -14,8 +14,12 @@ time, set up, stepped to completion and retired before the next starts.
 """
 import collections
 import hashlib
+import multiprocessing as mp
+import os
+import sys
 import time
 from dataclasses import dataclass, field
+from multiprocessing import shared_memory
 
 import numpy as np
 
@@ -121,14 +125,90 @@ class _Setup:
             self.P = None
 
 
+# --------------------------------------------------------------------------
+# Request preparation in worker processes.  The host facade holds one problem
+# per process, so problem generation (grid, weights, seeded initial state)
+# runs in a pool of processes; each hands back the grid descriptor blob and
+# the initial state through shared memory, and the server only creates the
+# device solver from them.
+_W = {"P": None, "key": None}
+
+
+def _worker_init(paths):
+    for p in paths:
+        if p not in sys.path:
+            sys.path.insert(0, p)
+
+
+def _worker_prepare(kind, seed, shape):
+    import gamera_op as gw
+    key = (kind, tuple(shape))
+    if _W["P"] is not None and _W["key"] == key:
+        _W["P"].reseed(seed)
+    else:
+        if _W["P"] is not None:
+            _W["P"].close()
+        _W["P"] = gw.Problem(kind, seed, *shape)
+        _W["key"] = key
+    P = _W["P"]
+    blob = P.desc_blob()
+    fields = [P.gas(f) for f in range(6)] + [P.gem(f) for f in range(3)]
+    nb = fields[0].nbytes
+    shm = shared_memory.SharedMemory(create=True, size=9 * nb)
+    for q, a in enumerate(fields):
+        np.ndarray(a.shape, dtype=np.float64, buffer=shm.buf, offset=q * nb)[...] = a
+    name = shm.name
+    shm.close()
+    return name, blob, fields[0].shape
+
+
+class Preparer:
+    """Pool of request-preparation processes.  Create it before the process
+    touches the GPU (worker processes are started fresh, never forked from a
+    process with a GPU context)."""
+
+    def __init__(self, workers=None):
+        n = workers or max(1, min(16, (os.cpu_count() or 2) - 1))
+        ctx = mp.get_context("spawn")
+        here = os.path.dirname(os.path.abspath(__file__))
+        self.pool = ctx.Pool(n, initializer=_worker_init, initargs=([here],))
+        self.workers = n
+
+    def submit(self, r):
+        return self.pool.apply_async(_worker_prepare, (r.kind, r.seed, r.shape))
+
+    @staticmethod
+    def solver(result, precision):
+        name, blob, shape = result
+        shm = shared_memory.SharedMemory(name=name)
+        try:
+            nb = int(np.prod(shape)) * 8
+            arrs = [np.ndarray(shape, dtype=np.float64, buffer=shm.buf, offset=q * nb)
+                    for q in range(9)]
+            S = go.Solver.from_parts(blob, arrs[:6], arrs[6:], precision)
+            del arrs
+        finally:
+            shm.close()
+            shm.unlink()
+        return S
+
+    def close(self):
+        self.pool.close()
+        self.pool.join()
+
+
 def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, digests=False,
-          realtime=True, log=None):
-    """Run the request stream through the continuous-batching loop.
+          realtime=True, log=None, preparer=None, lookahead=None):
+    """Run the request stream through the continuous-batching loop.  With a
+    Preparer, request set-up runs in its worker processes (up to `lookahead`
+    requests ahead); otherwise in this process, one request per quantum.
     Returns (results, wall seconds)."""
     reqs = sorted(requests, key=lambda r: r.arrival)
-    setup = _Setup()
+    setup = _Setup() if preparer is None else None
+    ahead = lookahead or (2 * preparer.workers if preparer is not None else 0)
     B = go.Batch()
-    queue = collections.deque()
+    queue = collections.deque()      # arrived, not yet being prepared
+    preparing = collections.deque()  # (request, async result) in arrival order
     active = []  # [request, solver, steps_left, admitted_time, bytes]
     pending = []
     results = []
@@ -140,25 +220,46 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     def now():
         return time.perf_counter() - t0
 
+    def admit(r, S):
+        nonlocal used
+        if tune:
+            S.autotune(2)  # measured once per grid shape, then cached in the library
+        b = S.device_bytes()
+        used += b
+        pending.append([r, S, r.steps, now(), b])
+
     try:
-        while nxt < len(reqs) or queue or active or pending:
+        while nxt < len(reqs) or queue or preparing or active or pending:
             tn = now()
             while nxt < len(reqs) and (reqs[nxt].arrival <= tn or not realtime):
                 queue.append(reqs[nxt])
                 nxt += 1
-            # admit (host set-up overlaps the quantum in flight); one request
-            # per pass while the GPU is busy, as many as fit when it is idle
-            while queue and len(active) + len(pending) < max_slots:
-                r = queue[0]
-                need = device_bytes_estimate(r.shape, r.precision)
-                if used + need > budget_bytes and (active or pending):
-                    break
-                queue.popleft()
-                S = setup.solver(r, tune)
-                used += S.device_bytes()
-                pending.append([r, S, r.steps, now(), S.device_bytes()])
-                if in_flight:
-                    break
+            if preparer is not None:
+                while queue and len(preparing) < ahead:
+                    r = queue.popleft()
+                    preparing.append((r, preparer.submit(r)))
+                # admit prepared requests in arrival order while memory and
+                # slots allow (the device upload overlaps the quantum in flight)
+                while preparing and preparing[0][1].ready() and \
+                        len(active) + len(pending) < max_slots:
+                    r = preparing[0][0]
+                    need = device_bytes_estimate(r.shape, r.precision)
+                    if used + need > budget_bytes and (active or pending):
+                        break
+                    _, res = preparing.popleft()
+                    admit(r, Preparer.solver(res.get(), r.precision))
+            else:
+                # host set-up in this process: one request per pass while the
+                # GPU is busy, as many as fit when it is idle
+                while queue and len(active) + len(pending) < max_slots:
+                    r = queue[0]
+                    need = device_bytes_estimate(r.shape, r.precision)
+                    if used + need > budget_bytes and (active or pending):
+                        break
+                    queue.popleft()
+                    admit(r, setup.solver(r, False))
+                    if in_flight:
+                        break
             if in_flight:
                 B.sync()
                 done = []
@@ -189,6 +290,8 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                 n = min(quantum, min(a[2] for a in active))
                 B.step_async(n)
                 in_flight = n
+            elif preparing:
+                preparing[0][1].wait(0.05)
             elif nxt < len(reqs) and not queue:
                 time.sleep(max(0.0, min(reqs[nxt].arrival - now(), 0.05)))
         wall = now()
@@ -196,7 +299,16 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
         B.close()
         for a in active + pending:
             a[1].close()
-        setup.close()
+        for r, res in preparing:  # release prepared but unused state
+            try:
+                name = res.get()[0]
+                shm = shared_memory.SharedMemory(name=name)
+                shm.close()
+                shm.unlink()
+            except Exception:
+                pass
+        if setup is not None:
+            setup.close()
     return results, wall
 
 

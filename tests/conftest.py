@@ -8,5 +8,16 @@ for p in (os.path.join(ROOT, "tests"), os.path.join(ROOT, "tests", "golden"),
         sys.path.insert(0, p)
 
 
+PREPARER = None
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP device solver)")
+    # The serving tests prepare requests in worker processes.  Start them now,
+    # before any test touches the GPU: workers are fresh processes and must
+    # never be started from a process that holds a GPU context.
+    expr = getattr(config.option, "markexpr", "") or ""
+    if "gpu" in expr and "not gpu" not in expr:
+        global PREPARER
+        import serve
+        PREPARER = serve.Preparer(workers=4)
