@@ -195,6 +195,7 @@ def run_serving(args, preparer):
         f"{args.serve_budget_gib} GiB")
     res, wall = serve.serve(reqs, budget_bytes=args.serve_budget_gib << 30, preparer=preparer)
     srv = serve.summarize(res, wall)
+    srv["loop_seconds"] = {k: round(v, 3) for k, v in serve.serve.last_stats.items()}
     log(f"bench: served {srv}")
     log("bench: plain loop on the same requests")
     res_p, wall_p = serve.serve_plain(reqs)

@@ -1354,6 +1354,19 @@ __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, con
   }
 }
 
+// reference layout (pitch NK) <-> padded device layout (pitch KP)
+__global__ __launch_bounds__(256) void k_pitch(const double *__restrict__ src,
+                                               double *__restrict__ dst, long rows, int nk,
+                                               int src_pitch, int dst_pitch) {
+  const long n = rows * nk;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const long r = q / nk;
+    const int k = (int)(q - r * nk);
+    dst[r * dst_pitch + k] = src[r * src_pitch + k];
+  }
+}
+
 // FP32 copy of n FP64 values (FP32-bulk form set-up)
 __global__ __launch_bounds__(256) void k_to32(const double *__restrict__ src,
                                               float *__restrict__ dst, size_t n) {
@@ -1415,6 +1428,7 @@ struct gk_solver {
   void *V[3];          // 2 face-velocity fields per direction
   void *E;             // 3 edge fields
   float *tables_f;     // FP32 copies of the reconstruction tables
+  double *stage;       // host transfer staging field (reference layout), lazily
   int precision;       // 0 FP64, 1 FP32-bulk
   size_t esz;          // bytes per intermediate value
   Scal *sc;
@@ -1623,6 +1637,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->tables);
   (void)hipFree(s->tables_f);
+  (void)hipFree(s->stage);
   (void)hipFree(s->itables);
   for (int b = 0; b < 2; b++) {
     (void)hipFree(s->X[b]);
@@ -1648,17 +1663,42 @@ extern "C" int gk_destroy(gk_solver *s) {
 
 size_t gk_device_bytes(const gk_solver *s) { return s ? s->bytes : 0; }
 
-// host reference layout (pitch NK) <-> device layout (pitch KP)
-static int copy_in(gk_solver *s, double *dst, const double *src) {
-  const size_t NK = s->g.N[2], rows = (size_t)s->g.N[0] * s->g.N[1];
-  HIPCHK(hipMemcpy2DAsync(dst, s->g.st[1] * sizeof(double), src, NK * sizeof(double),
-                          NK * sizeof(double), rows, hipMemcpyHostToDevice, s->stream));
+// Host reference layout (pitch NK) <-> device layout (pitch KP).  One
+// contiguous copy through a device staging field plus a re-pitch kernel: a
+// pitched copy from pageable memory would move one short row at a time.
+static int staging(gk_solver *s, double **out) {
+  if (!s->stage) {
+    const size_t n = (size_t)s->g.N[0] * s->g.N[1] * s->g.N[2];
+    if (hipMalloc((void **)&s->stage, n * sizeof(double)) != hipSuccess) return GK_ERR_NOMEM;
+    s->bytes += n * sizeof(double);
+  }
+  *out = s->stage;
   return GK_OK;
 }
+
+static int copy_in(gk_solver *s, double *dst, const double *src) {
+  double *st;
+  int rc = staging(s, &st);
+  if (rc != GK_OK) return rc;
+  const long rows = (long)s->g.N[0] * s->g.N[1];
+  const int NK = s->g.N[2];
+  HIPCHK(hipMemcpyAsync(st, src, rows * NK * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  k_pitch<<<2048, 256, 0, s->stream>>>(st, dst, rows, NK, NK, (int)s->g.st[1]);
+  HIPCHK(hipGetLastError());
+  return GK_OK;
+}
+
 static int copy_out(gk_solver *s, double *dst, const double *src) {
-  const size_t NK = s->g.N[2], rows = (size_t)s->g.N[0] * s->g.N[1];
-  HIPCHK(hipMemcpy2DAsync(dst, NK * sizeof(double), src, s->g.st[1] * sizeof(double),
-                          NK * sizeof(double), rows, hipMemcpyDeviceToHost, s->stream));
+  double *st;
+  int rc = staging(s, &st);
+  if (rc != GK_OK) return rc;
+  const long rows = (long)s->g.N[0] * s->g.N[1];
+  const int NK = s->g.N[2];
+  k_pitch<<<2048, 256, 0, s->stream>>>(src, st, rows, NK, (int)s->g.st[1], NK);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, st, rows * NK * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  // the staging field is reused by the next copy: finish this one first
+  HIPCHK(hipStreamSynchronize(s->stream));
   return GK_OK;
 }
 

@@ -135,6 +135,8 @@ _W = {"P": None, "key": None}
 
 
 def _worker_init(paths):
+    # one thread per worker: the pool, not OpenMP, provides the parallelism
+    os.environ["OMP_NUM_THREADS"] = "1"
     for p in paths:
         if p not in sys.path:
             sys.path.insert(0, p)
@@ -215,6 +217,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     used = 0
     nxt = 0
     in_flight = 0
+    stats = collections.Counter()
     t0 = time.perf_counter()
 
     def now():
@@ -247,7 +250,9 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                     if used + need > budget_bytes and (active or pending):
                         break
                     _, res = preparing.popleft()
+                    ta = time.perf_counter()
                     admit(r, Preparer.solver(res.get(), r.precision))
+                    stats["admit_s"] += time.perf_counter() - ta
             else:
                 # host set-up in this process: one request per pass while the
                 # GPU is busy, as many as fit when it is idle
@@ -257,11 +262,15 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                     if used + need > budget_bytes and (active or pending):
                         break
                     queue.popleft()
+                    ta = time.perf_counter()
                     admit(r, setup.solver(r, False))
+                    stats["admit_s"] += time.perf_counter() - ta
                     if in_flight:
                         break
             if in_flight:
+                ta = time.perf_counter()
                 B.sync()
+                stats["gpu_wait_s"] += time.perf_counter() - ta
                 done = []
                 for a in active:
                     a[2] -= in_flight
@@ -288,10 +297,15 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
             pending = []
             if active:
                 n = min(quantum, min(a[2] for a in active))
+                ta = time.perf_counter()
                 B.step_async(n)
+                stats["launch_s"] += time.perf_counter() - ta
+                stats["quanta"] += 1
                 in_flight = n
             elif preparing:
+                ta = time.perf_counter()
                 preparing[0][1].wait(0.05)
+                stats["prep_wait_s"] += time.perf_counter() - ta
             elif nxt < len(reqs) and not queue:
                 time.sleep(max(0.0, min(reqs[nxt].arrival - now(), 0.05)))
         wall = now()
@@ -309,6 +323,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                 pass
         if setup is not None:
             setup.close()
+    serve.last_stats = dict(stats)
     return results, wall
 
 

@@ -21,3 +21,10 @@ def pytest_configure(config):
         global PREPARER
         import serve
         PREPARER = serve.Preparer(workers=4)
+
+
+def pytest_unconfigure(config):
+    global PREPARER
+    if PREPARER is not None:
+        PREPARER.close()
+        PREPARER = None
