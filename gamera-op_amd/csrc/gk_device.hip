@@ -78,10 +78,16 @@ struct B0c {
   double lor[3];
 };
 
+// Device-resident scalars.  The fused reductions (next CFL dt, max |div B|)
+// land in NSLOT slots chosen by block coordinates, so that tens of thousands
+// of blocks do not all serialise on one atomic word; k_begin folds the slots.
+constexpr int NSLOT = 64;
+constexpr unsigned long long DBL_INF_BITS = 0x7FF0000000000000ULL;
+
 struct Scal {
   double dt, dt0, time, c;
-  unsigned long long dtmin;  // bit image of the next CFL dt (non-negative)
-  unsigned long long divb;   // bit image of max |div B|
+  unsigned long long smin[NSLOT];  // bit images of partial minima of the next CFL dt (>= 0)
+  unsigned long long smax[NSLOT];  // bit images of partial maxima of |div B|
   unsigned int nanflag;
   unsigned int pad;
 };
@@ -126,7 +132,7 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ void block_reduce(double dtc, double divb, Scal *sc) {
+__device__ __forceinline__ void block_reduce(double dtc, double divb, Scal *sc, int slot) {
   __shared__ double smin[8], smax[8];
   dtc = wave_min(dtc);
   divb = wave_max(divb);
@@ -144,37 +150,38 @@ __device__ __forceinline__ void block_reduce(double dtc, double divb, Scal *sc) 
       mn = fmin(mn, smin[n]);
       mx = fmax(mx, smax[n]);
     }
-    atomicMin(&sc->dtmin, (unsigned long long)__double_as_longlong(mn));
-    atomicMax(&sc->divb, (unsigned long long)__double_as_longlong(mx));
+    atomicMin(&sc->smin[slot & (NSLOT - 1)], (unsigned long long)__double_as_longlong(mn));
+    atomicMax(&sc->smax[slot & (NSLOT - 1)], (unsigned long long)__double_as_longlong(mx));
   }
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_begin(Scal *sc) {
-  double dt = __longlong_as_double((long long)sc->dtmin);
-  sc->dt = dt;
-  sc->time += dt;
-  sc->c = 0.5 * dt / sc->dt0;
-  sc->dt0 = dt;
-  sc->dtmin = 0x7FF0000000000000ULL;  // +inf
-  sc->divb = 0ULL;
+// Start of a step (one wave per grid): fold the reduction slots into dt, set
+// the AB2 factor and time, dt0 := dt, and reset the slots.
+__device__ __forceinline__ void begin_step(Scal *sc) {
+  const int q = threadIdx.x;  // NSLOT == 64 lanes
+  const double mn = wave_min(__longlong_as_double((long long)sc->smin[q]));
+  sc->smin[q] = DBL_INF_BITS;
+  sc->smax[q] = 0ULL;
+  if (q == 0) {
+    const double dt = mn;
+    sc->dt = dt;
+    sc->time += dt;
+    sc->c = 0.5 * dt / sc->dt0;
+    sc->dt0 = dt;
+  }
 }
 
-__global__ void k_begin_batch(Scal *const *scs, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  Scal *sc = scs[q];
-  double dt = __longlong_as_double((long long)sc->dtmin);
-  sc->dt = dt;
-  sc->time += dt;
-  sc->c = 0.5 * dt / sc->dt0;
-  sc->dt0 = dt;
-  sc->dtmin = 0x7FF0000000000000ULL;
-  sc->divb = 0ULL;
+__global__ __launch_bounds__(64) void k_begin(Scal *sc) { begin_step(sc); }
+
+__global__ __launch_bounds__(64) void k_begin_batch(Scal *const *scs, int n) {
+  if ((int)blockIdx.x < n) begin_step(scs[blockIdx.x]);
 }
 
-__global__ void k_set_dt0(Scal *sc) {
-  sc->dt0 = __longlong_as_double((long long)sc->dtmin);
+// dt0 := CFL dt of the initial state (prologue of a fresh start)
+__global__ __launch_bounds__(64) void k_set_dt0(Scal *sc) {
+  const double mn = wave_min(__longlong_as_double((long long)sc->smin[threadIdx.x]));
+  if (threadIdx.x == 0) sc->dt0 = mn;
 }
 
 // Arithmetic of the reconstruction, flux and EMF kernels in real type R.
@@ -1172,7 +1179,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     else
       bad = true;
   }
-  block_reduce(dtc, divb, a.sc);
+  block_reduce(dtc, divb, a.sc, bx * 31 + by * 7 + bz);
   if (bad) atomicOr(&a.sc->nanflag, 1u);
 }
 
@@ -1234,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
                  (Bf[2 * g.fs + c] + f2[2]) * geo3(G, GK_G_FACE_K, i, j, k)) / V;
     divb = fabs(db);
   }
-  block_reduce(dtc, divb, a.sc);
+  block_reduce(dtc, divb, a.sc, blockIdx.x * 31 + blockIdx.y * 7 + blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1793,7 +1800,10 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   HIPCHK(hipMemcpyAsync(&cur, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   Scal z{};
-  z.dtmin = 0x7FF0000000000000ULL;
+  for (int q = 0; q < NSLOT; q++) {
+    z.smin[q] = DBL_INF_BITS;
+    z.smax[q] = 0ULL;
+  }
   z.time = cur.time;  // keep the simulated time across prologues
   HIPCHK(hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
   DtArgs a;
@@ -1806,7 +1816,7 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   a.sc = s->sc;
   hipLaunchKernelGGL(k_dt, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
   if (s->ready == 1) {
-    hipLaunchKernelGGL(k_set_dt0, dim3(1), dim3(1), 0, s->stream, s->sc);
+    hipLaunchKernelGGL(k_set_dt0, dim3(1), dim3(NSLOT), 0, s->stream, s->sc);
   } else {
     HIPCHK(hipMemcpyAsync(&s->sc->dt0, &dt0_restart, sizeof(double), hipMemcpyHostToDevice,
                           s->stream));
@@ -2110,7 +2120,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
   const int total = gx * gy * gz;
   const dim3 g1((total + NXCD - 1) / NXCD * NXCD);  // decoded by xcd_block
   switch (L.kid) {
-    case SK_BEGIN: k_begin<<<1, 1, 0, st>>>(L.u.sc); break;
+    case SK_BEGIN: k_begin<<<1, NSLOT, 0, st>>>(L.u.sc); break;
     case SK_SWEEP_I: k_march<0, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
@@ -2130,7 +2140,7 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
                          hipStream_t st) {
   const dim3 blk(64, 4);
   switch (kid) {
-    case SK_BEGIN: k_begin_batch<<<(nwork + 63) / 64, 64, 0, st>>>((Scal *const *)args, nwork); break;
+    case SK_BEGIN: k_begin_batch<<<nwork, NSLOT, 0, st>>>((Scal *const *)args, nwork); break;
     case SK_SWEEP_I: k_march_batch<0, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
@@ -2245,8 +2255,13 @@ extern "C" int gk_get_scalars(gk_solver *s, double out[6]) {
   out[0] = h.dt;
   out[1] = h.dt0;
   out[2] = h.time;
-  memcpy(&out[3], &h.divb, sizeof(double));
-  memcpy(&out[4], &h.dtmin, sizeof(double));
+  unsigned long long mn = DBL_INF_BITS, mx = 0ULL;  // fold the reduction slots
+  for (int q = 0; q < NSLOT; q++) {
+    if (h.smin[q] < mn) mn = h.smin[q];
+    if (h.smax[q] > mx) mx = h.smax[q];
+  }
+  memcpy(&out[3], &mx, sizeof(double));
+  memcpy(&out[4], &mn, sizeof(double));
   out[5] = (double)h.nanflag;
   return GK_OK;
 }
