@@ -1444,6 +1444,7 @@ struct gk_solver {
   size_t bytes;
   int timing;
   int chunk;           // faces per thread in the marching sweeps
+  int emf_chunk;       // edges per thread in the marching EMF kernels
   struct gk_batch *batch;  // batch that owns the stepping, if any
   std::vector<std::string> names;
   std::vector<double> acc_ms;
@@ -1630,6 +1631,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     return fail(GK_ERR_DEVICE);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
   s->chunk = 32;
+  s->emf_chunk = 8;
   s->cur = 0;
   s->b_m1 = 0;
   s->b_0 = 1;
@@ -2036,15 +2038,16 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.V = s->V[D == 0 ? 1 : (D == 1 ? 2 : 0)];
     a.E = (char *)s->E + (size_t)D * s->fs * s->esz;
     a.sc = s->sc;
-    a.chunk = ch;
+    a.chunk = s->emf_chunk;
     for (int d = 0; d < 3; d++) {
       a.lo[d] = S[d];
       a.hi[d] = Ee[d] + (d == D ? 0 : 1);
     }
     const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
+    const int ce = s->emf_chunk;
     L.kid = SK_EMF_I + D;
     L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[RA] - a.lo[RA] + 4) / 4,
-                  (a.hi[MA] - a.lo[MA] + ch) / ch);
+                  (a.hi[MA] - a.lo[MA] + ce) / ce);
     L.u.em = a;
     out.push_back(L);
   }
@@ -2326,9 +2329,13 @@ extern "C" int gk_copy_calibrate(size_t bytes, int reps, double *ms) {
 
 extern "C" int gk_set_tuning(gk_solver *s, int param, int value) {
   if (!s) return GK_ERR_ARG;
+  if (value < 1 || value > 4096) return GK_ERR_ARG;
   if (param == GK_TUNE_CHUNK) {
-    if (value < 1 || value > 4096) return GK_ERR_ARG;
     s->chunk = value;
+    return GK_OK;
+  }
+  if (param == GK_TUNE_EMF_CHUNK) {
+    s->emf_chunk = value;
     return GK_OK;
   }
   return GK_ERR_ARG;
@@ -2337,6 +2344,7 @@ extern "C" int gk_set_tuning(gk_solver *s, int param, int value) {
 extern "C" int gk_get_tuning(const gk_solver *s, int param) {
   if (!s) return GK_ERR_ARG;
   if (param == GK_TUNE_CHUNK) return s->chunk;
+  if (param == GK_TUNE_EMF_CHUNK) return s->emf_chunk;
   return GK_ERR_ARG;
 }
 
@@ -2638,7 +2646,10 @@ struct TuneKey {
     return ni == o.ni && nj == o.nj && nk == o.nk && coord == o.coord && prec == o.prec;
   }
 };
-static std::vector<std::pair<TuneKey, int>> g_tune_cache;
+struct TunePick {
+  int chunk, emf_chunk;
+};
+static std::vector<std::pair<TuneKey, TunePick>> g_tune_cache;
 
 extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   if (!s || trial_steps < 1) return GK_ERR_ARG;
@@ -2646,8 +2657,9 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   const TuneKey key{s->ni, s->nj, s->nk, s->spherical, s->precision};
   for (auto &e : g_tune_cache)
     if (e.first == key) {
-      s->chunk = e.second;
-      if (best) *best = e.second;
+      s->chunk = e.second.chunk;
+      s->emf_chunk = e.second.emf_chunk;
+      if (best) *best = e.second.chunk;
       return GK_OK;
     }
   const size_t fs = s->fs;
@@ -2681,32 +2693,49 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   };
   int rc = snap(false);
   const int maxlen = std::max(s->ni, std::max(s->nj, s->nk)) + 1;
-  const int cand[] = {8, 16, 32, 64, 128, 256};
-  int pick = s->chunk;
-  float best_ms = 1e30f;
+  const int keep_timing = s->timing;
+  s->timing = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rc == GK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = GK_ERR_DEVICE;
-  const int keep_timing = s->timing;
-  s->timing = 0;
+  // time trial_steps steps of the current shape (after one warm-up step)
+  auto trial = [&](float &ms) -> int {
+    one_step(s);
+    (void)hipEventRecord(e0, s->stream);
+    for (int t = 0; t < trial_steps; t++) one_step(s);
+    (void)hipEventRecord(e1, s->stream);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return GK_ERR_DEVICE;
+    return snap(true);
+  };
+  // march length of the sweeps, then of the edge-field kernels
+  const int cand[] = {8, 16, 32, 64, 128, 256};
+  const int cand_e[] = {2, 4, 8, 16, 32};
+  int pick = s->chunk, pick_e = s->emf_chunk;
+  float best_ms = 1e30f;
   for (int c : cand) {
     if (rc != GK_OK) break;
     if (c > 2 * maxlen && c != cand[0]) break;
     s->chunk = c;
-    one_step(s);  // warm-up (first launch of this shape)
-    (void)hipEventRecord(e0, s->stream);
-    for (int t = 0; t < trial_steps; t++) one_step(s);
-    (void)hipEventRecord(e1, s->stream);
     float ms = 0.0f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-      rc = GK_ERR_DEVICE;
-      break;
-    }
+    if ((rc = trial(ms)) != GK_OK) break;
     if (ms < best_ms) {
       best_ms = ms;
       pick = c;
     }
-    rc = snap(true);
+  }
+  s->chunk = pick;
+  best_ms = 1e30f;
+  for (int c : cand_e) {
+    if (rc != GK_OK) break;
+    if (c > 2 * maxlen && c != cand_e[0]) break;
+    s->emf_chunk = c;
+    float ms = 0.0f;
+    if ((rc = trial(ms)) != GK_OK) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      pick_e = c;
+    }
   }
   s->timing = keep_timing;
   if (e0) (void)hipEventDestroy(e0);
@@ -2714,7 +2743,8 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   (void)hipFree(save);
   if (rc != GK_OK) return rc;
   s->chunk = pick;
-  g_tune_cache.push_back({key, pick});
+  s->emf_chunk = pick_e;
+  g_tune_cache.push_back({key, TunePick{pick, pick_e}});
   if (best) *best = pick;
   return GK_OK;
 }

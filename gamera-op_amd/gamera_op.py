@@ -175,6 +175,7 @@ def copy_calibrate(nbytes, reps=20):
 
 
 TUNE_CHUNK = 0
+TUNE_EMF_CHUNK = 1
 
 
 class GkError(RuntimeError):
