@@ -195,6 +195,14 @@ __device__ __forceinline__ double qsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double qpow(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ float qpow(float x, float y) { return powf(x, y); }
+// a * b + c: in FP64 a rounded product, then a rounded sum (the reference's
+// operations; c + a * b gives the same bits); in FP32-bulk one fused
+// multiply-add (one rounding)
+__device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// half-step value x + c (x - x_prev) of the AB2 predictor
+template <typename R>
+__device__ __forceinline__ R half_step(R x, R xp, R c) { return madd(c, x - xp, x); }
 
 // reconstruction table of sweep axis A at face f for a point with
 // coordinates idx (the two transverse coordinates select the class)
@@ -228,9 +236,10 @@ __device__ __forceinline__ R pdm_limit(R f_itp, R fm1, R f0, R fp1, R c1, R c2) 
   df0 = fabs(df0);
   R q0 = fabs(s0 + s1);
   R dfl = f - f0;
-  R df0c = fabs(c1 * f0 + c2 * fm1) / R(2);
-  df0c = R(0.5) * df0c + R(0.5) * df0;
-  return f - s1 * fmax(R(0), fabs(dfl) - q0 * df0c);
+  R df0c = fabs(madd(c2, fm1, c1 * f0)) / R(2);
+  df0c = madd(R(0.5), df0, R(0.5) * df0c);
+  // f - s1 max(0, |dfl| - q0 df0c), with x - y z written x + (-y) z
+  return madd(-s1, fmax(R(0), madd(-q0, df0c, fabs(dfl))), f);
 }
 
 // Left/right states at a face from 8 stencil cells (f-4 .. f+3): xh are the
@@ -240,8 +249,8 @@ __device__ __forceinline__ void pdm_lr(const R xh[8], const R xn[8], const R *T,
   R fl = R(0), fr = R(0);
 #pragma unroll
   for (int n = 0; n < 7; n++) {
-    fl += xh[n] * T[n];
-    fr += xh[7 - n] * T[7 + n];
+    fl = madd(xh[n], T[n], fl);
+    fr = madd(xh[7 - n], T[7 + n], fr);
   }
   L = pdm_limit(fl, xn[2], xn[3], xn[4], T[14], T[15]);
   Rr = pdm_limit(fr, xn[5], xn[4], xn[3], T[16], T[17]);
@@ -258,7 +267,7 @@ __device__ __forceinline__ void recon(const S *__restrict__ p, const S *__restri
     long o = c + (n - 4) * sd;
     R x = (R)p[o];
     xn[n] = x;
-    xh[n] = q ? x + cc * (x - (R)q[o]) : x;
+    xh[n] = q ? half_step(x, (R)q[o], cc) : x;
   }
   pdm_lr(xh, xn, T, L, Rr);
 }
@@ -289,16 +298,20 @@ __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R b
   const R vl[3] = {L[1], L[2], L[3]}, vr[3] = {Rs[1], Rs[2], Rs[3]};
   R csl = qsqrt(qdiv(G * pl, rl)), csr = qsqrt(qdiv(G * pr, rr));
   R cel = qsqrt(qdiv(G * psl, rl)), cer = qsqrt(qdiv(G * psr, rr));
-  R Btl = (bl[0] + b0[0]) * (bl[0] + b0[0]) + (bl[1] + b0[1]) * (bl[1] + b0[1]) +
-          (bl[2] + b0[2]) * (bl[2] + b0[2]);
-  R Btr = (br[0] + b0[0]) * (br[0] + b0[0]) + (br[1] + b0[1]) * (br[1] + b0[1]) +
-          (br[2] + b0[2]) * (br[2] + b0[2]);
+  // sums of products are written with madd in the reference's order:
+  // a + b * c as madd(b, c, a), a - b * c as madd(-b, c, a)
+  const R tl[3] = {bl[0] + b0[0], bl[1] + b0[1], bl[2] + b0[2]};
+  const R tr[3] = {br[0] + b0[0], br[1] + b0[1], br[2] + b0[2]};
+  R Btl = madd(tl[2], tl[2], madd(tl[1], tl[1], tl[0] * tl[0]));
+  R Btr = madd(tr[2], tr[2], madd(tr[1], tr[1], tr[0] * tr[0]));
   R Val = qsqrt(qdiv(Btl, rl)), Var = qsqrt(qdiv(Btr, rr));
-  R Vel = qdiv(Val * CA, qsqrt(Val * Val + CA * CA));
-  R Ver = qdiv(Var * CA, qsqrt(Var * Var + CA * CA));
+  R Vel = qdiv(Val * CA, qsqrt(madd(Val, Val, CA * CA)));
+  R Ver = qdiv(Var * CA, qsqrt(madd(Var, Var, CA * CA)));
   R vnl = vl[D], vnr = vr[D];
-  R VF = fmax(fabs(vnl) + qsqrt(csl * csl + Vel * Vel), fabs(vnr) + qsqrt(csr * csr + Ver * Ver));
-  R VFe = fmax(fabs(vnl) + qsqrt(cel * cel + Vel * Vel), fabs(vnr) + qsqrt(cer * cer + Ver * Ver));
+  R VF = fmax(fabs(vnl) + qsqrt(madd(Vel, Vel, csl * csl)),
+              fabs(vnr) + qsqrt(madd(Ver, Ver, csr * csr)));
+  R VFe = fmax(fabs(vnl) + qsqrt(madd(Vel, Vel, cel * cel)),
+               fabs(vnr) + qsqrt(madd(Ver, Ver, cer * cer)));
   R rvl[3], rvr[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) {
@@ -306,38 +319,40 @@ __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R b
     rvr[m] = rr * vr[m];
   }
   const R h = R(0.5);
-  R engl = h * rl * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) + qdiv(pl, R(GAMMA - 1.0));
-  R engr = h * rr * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) + qdiv(pr, R(GAMMA - 1.0));
+  const R v2l = madd(vl[2], vl[2], madd(vl[1], vl[1], vl[0] * vl[0]));
+  const R v2r = madd(vr[2], vr[2], madd(vr[1], vr[1], vr[0] * vr[0]));
+  R engl = madd(h * rl, v2l, qdiv(pl, R(GAMMA - 1.0)));
+  R engr = madd(h * rr, v2r, qdiv(pr, R(GAMMA - 1.0)));
   R Sl = entropy_over(psl, rl), Sr = entropy_over(psr, rr);
-  Fo[0] = h * (rvl[D] + rvr[D] - VF * (rr - rl));
+  Fo[0] = h * madd(-VF, rr - rl, rvl[D] + rvr[D]);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     if (m == D)
-      Fo[1 + m] = h * (rvl[m] * vnl + pl + rvr[m] * vnr + pr - VF * (rvr[m] - rvl[m]));
+      Fo[1 + m] = h * madd(-VF, rvr[m] - rvl[m], madd(rvr[m], vnr, madd(rvl[m], vnl, pl)) + pr);
     else
-      Fo[1 + m] = h * (rvl[m] * vnl + rvr[m] * vnr - VF * (rvr[m] - rvl[m]));
+      Fo[1 + m] = h * madd(-VF, rvr[m] - rvl[m], madd(rvr[m], vnr, rvl[m] * vnl));
   }
-  Fo[4] = h * (engl * vnl + pl * vnl + engr * vnr + pr * vnr - VF * (engr - engl));
-  Fo[5] = h * (Sl * vnl + Sr * vnr - VFe * (Sr - Sl));
+  Fo[4] = h * madd(-VF, engr - engl, madd(pr, vnr, madd(engr, vnr, madd(pl, vnl, engl * vnl))));
+  Fo[5] = h * madd(-VFe, Sr - Sl, madd(Sr, vnr, Sl * vnl));
 
   const R b0n = b0[3];
-  R bsql = (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]) +
-           2 * (bl[0] * b0[0] + bl[1] * b0[1] + bl[2] * b0[2]);
-  R bsqr = (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]) +
-           2 * (br[0] * b0[0] + br[1] * b0[1] + br[2] * b0[2]);
+  R bsql = madd(R(2), madd(bl[2], b0[2], madd(bl[1], b0[1], bl[0] * b0[0])),
+                madd(bl[2], bl[2], madd(bl[1], bl[1], bl[0] * bl[0])));
+  R bsqr = madd(R(2), madd(br[2], b0[2], madd(br[1], b0[1], br[0] * b0[0])),
+                madd(br[2], br[2], madd(br[1], br[1], br[0] * br[0])));
   R Bsq = qdiv(qdiv(Btl + Btr, R(4)), CA);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     R sp, sn;
     if (m == D) {
-      sp = h * (h * bsql - (bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = h * (h * bsqr - (br[m] + b0[m]) * Bn - br[m] * b0n);
+      sp = h * madd(-bl[m], b0n, madd(-tl[m], Bn, h * bsql));
+      sn = h * madd(-br[m], b0n, madd(-tr[m], Bn, h * bsqr));
     } else {
-      sp = h * (-(bl[m] + b0[m]) * Bn - bl[m] * b0n);
-      sn = h * (-(br[m] + b0[m]) * Bn - br[m] * b0n);
+      sp = h * madd(-bl[m], b0n, -tl[m] * Bn);
+      sn = h * madd(-br[m], b0n, -tr[m] * Bn);
     }
     R ms = sp + sn;
-    Fo[6 + m] = ms - Bsq * (vr[m] - vl[m]);
+    Fo[6 + m] = madd(-Bsq, vr[m] - vl[m], ms);
   }
 }
 
@@ -479,7 +494,7 @@ __device__ __forceinline__ void march_load(Window<NF, R> &w, const S *const P[NF
 #pragma unroll
   for (int m = 0; m < NF; m++) {
     const R x = (R)P[m][o];
-    w.h[m][slot] = x + cc * (x - (R)Q[m][o]);
+    w.h[m][slot] = half_step(x, (R)Q[m][o], cc);
     if (slot >= 2) w.x[m][slot - 2] = x;
   }
 }
@@ -500,8 +515,8 @@ __device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R 
   R fl = R(0), fr = R(0);
 #pragma unroll
   for (int n = 0; n < 7; n++) {
-    fl += w.h[m][n] * T[n];
-    fr += w.h[m][7 - n] * T[7 + n];
+    fl = madd(w.h[m][n], T[n], fl);
+    fr = madd(w.h[m][7 - n], T[7 + n], fr);
   }
   // limiter cells: x[0] = f-2, x[1] = f-1, x[2] = f, x[3] = f+1
   L = pdm_limit(fl, w.x[m][0], w.x[m][1], w.x[m][2], T[14], T[15]);
@@ -637,11 +652,11 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
   const long ob = rowb + min(k0 + 60 + tx, g.N[2] - 1);
   auto stage = [&](int m, const R *P, const R *Q) {
     const R xa = (R)P[oa], pa = (R)Q[oa];
-    sh[m][ty][tx] = xa + cc * (xa - pa);
+    sh[m][ty][tx] = half_step(xa, pa, cc);
     sx[m][ty][tx] = xa;
     if (tx < KSPAN - 64) {
       const R xb = (R)P[ob], pb = (R)Q[ob];
-      sh[m][ty][64 + tx] = xb + cc * (xb - pb);
+      sh[m][ty][64 + tx] = half_step(xb, pb, cc);
       sx[m][ty][64 + tx] = xb;
     }
   };
@@ -668,8 +683,8 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
     R fl = R(0), fr = R(0);
 #pragma unroll
     for (int n = 0; n < 7; n++) {
-      fl += sh[m][ry][b + n] * T[n];
-      fr += sh[m][ry][b + 7 - n] * T[7 + n];
+      fl = madd(sh[m][ry][b + n], T[n], fl);
+      fr = madd(sh[m][ry][b + 7 - n], T[7 + n], fr);
     }
     L = pdm_limit(fl, sx[m][ry][b + 2], sx[m][ry][b + 3], sx[m][ry][b + 4], T[14], T[15]);
     Rr = pdm_limit(fr, sx[m][ry][b + 5], sx[m][ry][b + 4], sx[m][ry][b + 3], T[16], T[17]);
@@ -729,27 +744,29 @@ __device__ __forceinline__ R emf_formula(R va, R vb, R b1l, R b1r, R b2l, R b2r,
   const R CA = (R)CAd, h = R(0.5);
   const R e0[3] = {(R)e0d[0], (R)e0d[1], (R)e0d[2]};
   R e, dv;
+  // a + b * c as madd(b, c, a), a - b * c as madd(-b, c, a): the reference's
+  // operations and order
   if (D == 0) {
-    R bja = h * (b1l + b1r) + e0[1];
-    R bka = h * (b2l + b2r) + e0[2];
-    e = -(va * bka - vb * bja);
-    dv = qsqrt(qdiv(bja * bja + bka * bka, R(0.25) * rsum));
-    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(va * va + vb * vb);
-    e += h * dv * (b1l + b2r - b1r - b2l);
+    R bja = madd(h, b1l + b1r, e0[1]);
+    R bka = madd(h, b2l + b2r, e0[2]);
+    e = -madd(-vb, bja, va * bka);
+    dv = qsqrt(qdiv(madd(bka, bka, bja * bja), R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(madd(dv, dv, CA * CA))) + qsqrt(madd(vb, vb, va * va));
+    e = madd(h * dv, b1l + b2r - b1r - b2l, e);
   } else if (D == 1) {
-    R bka = h * (b1l + b1r) + e0[2];
-    R bia = h * (b2l + b2r) + e0[0];
-    e = -(va * bia - vb * bka);
-    dv = qsqrt(qdiv(bka * bka + bia * bia, R(0.25) * rsum));
-    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(vb * vb + va * va);
-    e += h * dv * (b1l + b2r - b1r - b2l);
+    R bka = madd(h, b1l + b1r, e0[2]);
+    R bia = madd(h, b2l + b2r, e0[0]);
+    e = -madd(-vb, bka, va * bia);
+    dv = qsqrt(qdiv(madd(bia, bia, bka * bka), R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(madd(dv, dv, CA * CA))) + qsqrt(madd(va, va, vb * vb));
+    e = madd(h * dv, b1l + b2r - b1r - b2l, e);
   } else {
-    R bia = h * (b1l + b1r) + e0[0];
-    R bja = h * (b2l + b2r) + e0[1];
-    e = -(va * bja - vb * bia);
-    dv = qsqrt(qdiv(bia * bia + bja * bja, R(0.25) * rsum));
-    dv = qdiv(dv * CA, qsqrt(dv * dv + CA * CA)) + qsqrt(vb * vb + va * va);
-    e += h * dv * (b1l + b2r - b1r - b2l);
+    R bia = madd(h, b1l + b1r, e0[0]);
+    R bja = madd(h, b2l + b2r, e0[1]);
+    e = -madd(-vb, bia, va * bja);
+    dv = qsqrt(qdiv(madd(bja, bja, bia * bia), R(0.25) * rsum));
+    dv = qdiv(dv * CA, qsqrt(madd(dv, dv, CA * CA))) + qsqrt(madd(va, va, vb * vb));
+    e = madd(h * dv, b1l + b2r - b1r - b2l, e);
   }
   return e;
 }
@@ -784,8 +801,8 @@ __device__ __forceinline__ void lr_self(const Window1<R> &w, const R *T, R &L, R
   R fl = R(0), fr = R(0);
 #pragma unroll
   for (int n = 0; n < 7; n++) {
-    fl += w.h[n] * T[n];
-    fr += w.h[7 - n] * T[7 + n];
+    fl = madd(w.h[n], T[n], fl);
+    fr = madd(w.h[7 - n], T[7 + n], fr);
   }
   L = pdm_limit(fl, w.h[2], w.h[3], w.h[4], T[14], T[15]);
   Rr = pdm_limit(fr, w.h[5], w.h[4], w.h[3], T[16], T[17]);
@@ -832,7 +849,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
   auto rh = [&](long o) {
     R x = (R)X[o];
-    return x + cc * (x - (R)Xp[o]);
+    return half_step(x, (R)Xp[o], cc);
   };
   const R *Tm = rtab<MA, R>(a.geo, f0, idx);  // windowed reconstructions
   const long tstep = (long)a.geo.ncls[MA][0] * a.geo.ncls[MA][1] * GK_RECW_STRIDE;
