@@ -407,17 +407,23 @@ struct SweepArgs {
   int chunk;             // faces per thread in the marching sweeps
 };
 
-// Direct sweep for the velocity-only strips of the j sweep (k in the four
-// ghost layers on either side): each thread reads its eight stencil cells;
-// a wave covers the 4 strip cells of 16 consecutive j rows.
+// Direct sweep for narrow k ranges: the velocity-only strips of the j sweep
+// (k in the four ghost layers on either side) and the k faces left over when
+// the k extent is not a multiple of the 64-face tiles of k_sweepk.  Each
+// thread reads its eight stencil cells; the points of the launch box are
+// walked k fastest, then j, then i, 256 per block.
 template <int D, typename R>
 __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
-  (void)bx;
-  const int k = a.lo[2] + (threadIdx.x & (NGH - 1));
-  const int j = a.lo[1] + by * 64 + threadIdx.y * 16 + (threadIdx.x >> 2);
-  const int i = a.lo[0] + bz;
-  if (k > a.hi[2] || j > a.hi[1]) return;
+  (void)by;
+  (void)bz;
+  const int W = a.hi[2] - a.lo[2] + 1, H = a.hi[1] - a.lo[1] + 1;
+  const int p = bx * 256 + threadIdx.y * 64 + threadIdx.x;
+  const int t = p / W;
+  const int k = a.lo[2] + (p - t * W);
+  const int j = a.lo[1] + t % H;
+  const int i = a.lo[0] + t / H;
+  if (i > a.hi[0]) return;
   const Grid &g = a.g;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const long sd = g.st[D];
@@ -482,6 +488,20 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
 // limiter need: each face loads one new cell instead of eight.  Rows outside
 // [flo, fhi] of RA only form the two face-averaged velocities.  The per-face
 // arithmetic is that of k_sweep.
+// Points of a marching launch: the (row, k) plane of the launch box is walked
+// row by row, k fastest, 256 points per block (bx), so a k extent that is not
+// a multiple of 64 (257 faces or edges, small grids) does not leave waves
+// mostly empty; the lanes of a wave may span two rows.
+__device__ __forceinline__ bool row_point(int klo, int khi, int rlo, int rhi, int bx, int &k,
+                                          int &r) {
+  const int wk = khi - klo + 1;
+  const int p = bx * 256 + threadIdx.y * 64 + threadIdx.x;
+  const int q = p / wk;
+  r = rlo + q;
+  k = klo + (p - q * wk);
+  return r <= rhi;
+}
+
 template <int NF, typename R>
 struct Window {
   R h[NF][8];  // half-step values at cells f-4 .. f+3
@@ -529,9 +549,9 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
   const int chunk = a.chunk;
   constexpr int RA = D == 0 ? 1 : 0;  // row axis
   const Grid &g = a.g;
-  const int k = a.lo[2] + bx * 64 + threadIdx.x;
-  const int r = a.lo[RA] + by * 4 + threadIdx.y;
-  if (k > a.hi[2] || r > a.hi[RA]) return;
+  (void)by;
+  int k, r;
+  if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
   const int f0 = a.lo[D] + bz * chunk;
   const int f1 = min(f0 + chunk, a.hi[D] + 1);
   int idx[3];
@@ -623,11 +643,12 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
 }
 
 // k sweep (D = 2) with the stencil staged in LDS: a block covers 64 faces
-// along k times KROWS rows along j at one i; each of the reconstructed fields
+// along k times KROWS rows along j at one i (one wave per block: no wave waits
+// at the barrier for another); each of the reconstructed fields
 // is loaded once per block (72 cells per row: 64 faces + 8 stencil halo) as
 // half-step and current values.  Planes i outside [flo0, fhi0] only form the
 // face-averaged velocities (v3, v1).
-constexpr int KROWS = 4;
+constexpr int KROWS = 1;
 constexpr int KSPAN = 72;
 constexpr int CELL_CHUNK = 1;  // cells each thread of the cell update walks along i
 
@@ -829,9 +850,9 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   constexpr int O = P == MA ? Q : P;  // density axis other than the march axis
   constexpr bool VW = A2 == MA;       // v and F1 windowed
   const Grid &g = a.g;
-  const int k = a.lo[2] + bx * 64 + threadIdx.x;
-  const int r = a.lo[RA] + by * 4 + threadIdx.y;
-  if (k > a.hi[2] || r > a.hi[RA]) return;
+  (void)by;
+  int k, r;
+  if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
   const int f0 = a.lo[MA] + bz * chunk;
   const int f1 = min(f0 + chunk, a.hi[MA] + 1);
   int idx[3];
@@ -1899,11 +1920,11 @@ void resolve_timers(gk_solver *s) {
 // path (launched one by one) and the batched path (launches of many grids
 // merged per stage into one work-list launch).
 enum StageKernel {
-  SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_EMF_I, SK_EMF_J, SK_EMF_K,
-  SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
+  SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
+  SK_EMF_K, SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
 };
 static const char *const sk_name[SK_COUNT] = {
-    "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_emf_i",
+    "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
     "k_emf_j", "k_emf_k", "k_ct", "k_cell", "k_ghost", "k_mags"};
 
 union StageArgs {
@@ -1919,7 +1940,8 @@ union StageArgs {
 static size_t stage_arg_size(int kid) {
   switch (kid) {
     case SK_BEGIN: return sizeof(Scal *);
-    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: return sizeof(SweepArgs);
+    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
+      return sizeof(SweepArgs);
     case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return sizeof(EmfArgs);
     case SK_CT: return sizeof(CtArgs);
     case SK_CELL: return sizeof(CellArgs);
@@ -1931,7 +1953,8 @@ static size_t stage_arg_size(int kid) {
 static const void *stage_arg_ptr(int kid, const StageArgs &u) {
   switch (kid) {
     case SK_BEGIN: return &u.sc;
-    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: return &u.sw;
+    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
+      return &u.sw;
     case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return &u.em;
     case SK_CT: return &u.ct;
     case SK_CELL: return &u.ce;
@@ -1952,6 +1975,17 @@ struct Rot {
 };
 static Rot rot_of(const gk_solver *s) { return Rot{s->cur, s->b_m1, s->b_0, s->b_n}; }
 static Rot rot_next(Rot r) { return Rot{1 - r.cur, r.b_0, r.b_n, r.b_m1}; }
+
+// blocks of 256 points over the (row, k) plane of a marching launch
+static int row_blocks(int klo, int khi, int rlo, int rhi) {
+  return ((khi - klo + 1) * (rhi - rlo + 1) + 255) / 256;
+}
+
+// blocks of 256 points over the launch box of the direct sweep
+static int point_blocks(const SweepArgs &v) {
+  const long n = (long)(v.hi[2] - v.lo[2] + 1) * (v.hi[1] - v.lo[1] + 1) * (v.hi[0] - v.lo[0] + 1);
+  return (int)((n + 255) / 256);
+}
 
 static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out) {
   out.clear();
@@ -2009,7 +2043,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.hi[X2] = g.N[X2] - 2;
     if (D == 0) {
       L.kid = SK_SWEEP_I;
-      L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + 4) / 4,
+      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[1], a.hi[1]), 1,
                     (a.hi[0] - a.lo[0] + ch) / ch);
       L.u.sw = a;
       out.push_back(L);
@@ -2018,7 +2052,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       m.lo[2] = S[2];
       m.hi[2] = Ee[2];
       L.kid = SK_SWEEP_J;
-      L.grid = dim3((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[0] - m.lo[0] + 4) / 4,
+      L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1,
                     (m.hi[1] - m.lo[1] + ch) / ch);
       L.u.sw = m;
       out.push_back(L);
@@ -2028,16 +2062,31 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
         v.lo[2] = side == 0 ? 0 : Ee[2] + 1;
         v.hi[2] = side == 0 ? S[2] - 1 : g.N[2] - 2;
         L.kid = SK_VSTRIP_J;  // strip width is NGH = 4 on both sides
-        L.grid = dim3(1, (v.hi[1] - v.lo[1] + 64) / 64, v.hi[0] - v.lo[0] + 1);
+        L.grid = dim3(point_blocks(v), 1, 1);
         L.u.sw = v;
         out.push_back(L);
       }
     } else {
+      // whole 64-face tiles in k_sweepk; a remainder of fewer than 32 faces
+      // per row goes to the direct sweep instead of a mostly empty tile
+      const int wk = a.hi[2] - a.lo[2] + 1;
+      const int rem = wk % 64;
+      SweepArgs m = a;
+      const bool split = wk >= 64 && rem > 0 && rem < 32;
+      if (split) m.hi[2] = a.hi[2] - rem;
       L.kid = SK_SWEEP_K;
-      L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[1] - a.lo[1] + KROWS) / KROWS,
-                    a.hi[0] - a.lo[0] + 1);
-      L.u.sw = a;
+      L.grid = dim3((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[1] - m.lo[1] + KROWS) / KROWS,
+                    m.hi[0] - m.lo[0] + 1);
+      L.u.sw = m;
       out.push_back(L);
+      if (split) {
+        SweepArgs v = a;
+        v.lo[2] = m.hi[2] + 1;
+        L.kid = SK_KREM;
+        L.grid = dim3(point_blocks(v), 1, 1);
+        L.u.sw = v;
+        out.push_back(L);
+      }
     }
   }
   // edge electric fields
@@ -2063,7 +2112,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
     const int ce = s->emf_chunk;
     L.kid = SK_EMF_I + D;
-    L.grid = dim3((a.hi[2] - a.lo[2] + 64) / 64, (a.hi[RA] - a.lo[RA] + 4) / 4,
+    L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA]), 1,
                   (a.hi[MA] - a.lo[MA] + ce) / ce);
     L.u.em = a;
     out.push_back(L);
@@ -2145,6 +2194,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64, KROWS), 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_KREM: k_sweep<2, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_EMF_I: k_emf_march<0, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_EMF_J: k_emf_march<1, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_EMF_K: k_emf_march<2, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
@@ -2165,6 +2215,7 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
     case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64, KROWS), 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_KREM: k_sweep_batch<2, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_EMF_I: k_emf_march_batch<0, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_EMF_J: k_emf_march_batch<1, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_EMF_K: k_emf_march_batch<2, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;

@@ -556,7 +556,10 @@ int solve(int nt) {
     /* initial state after ghost fill and BCs (src/solver_solve.c:44-53) */
     hdf_seq_num = -1;
     log_seq_num = 0;
-    if (device_fetch(0) != GK_OK) return -1;
+    if ((rc = device_fetch(0)) != GK_OK) {
+      log_error("Error: device state download failed (%d).", rc);
+      return -1;
+    }
     io->dump(BASE_DAT_NAME);
     io->extra(BASE_EXT_NAME);
     last_dump_time = 0;
@@ -564,19 +567,31 @@ int solve(int nt) {
   const int per_step = dumps || (time_stop < 1e300 && time_stop > 0.0);
   if (!per_step) {
     rc = gk_step(g_dev, nt);
-    if (rc != GK_OK) return -1;
+    if (rc != GK_OK) {
+      log_error("Error: device step failed (%d).", rc);
+      return -1;
+    }
   } else {
     for (int n = 1; n <= nt; n++) {
       rc = gk_step(g_dev, 1);
-      if (rc != GK_OK) return -1;
+      if (rc != GK_OK) {
+        log_error("Error: device step %d failed (%d).", n, rc);
+        return -1;
+      }
       double sc[6];
-      if (gk_get_scalars(g_dev, sc) != GK_OK) return -1;
+      if ((rc = gk_get_scalars(g_dev, sc)) != GK_OK) {
+        log_error("Error: device scalars failed (%d).", rc);
+        return -1;
+      }
       if (sc[5] != 0.0) log_error("non-finite state detected at step %d", n);
       time_sim = sc[2];
       dt = sc[0];
       divB_max = sc[3];
       if (dumps && time_sim >= output_interval * (hdf_seq_num + 1)) {
-        if (gkh_device_fetch() != GK_OK) return -1;
+        if ((rc = gkh_device_fetch()) != GK_OK) {
+          log_error("Error: device state download failed (%d).", rc);
+          return -1;
+        }
         io->dump(BASE_DAT_NAME);
         log_info("Output data at time %.15f", time_sim);
         last_dump_time = time_sim;
@@ -593,7 +608,10 @@ int solve(int nt) {
     }
   }
   rc = gkh_device_fetch();
-  if (rc != GK_OK) return -1;
+  if (rc != GK_OK) {
+    log_error("Error: device state download failed (%d).", rc);
+    return -1;
+  }
   if (dumps && last_dump_time != time_sim) io->dump(BASE_DAT_NAME);
   return 0;
 }
@@ -628,6 +646,9 @@ int gkh_main(const char *config_file, int nt) {
   Source1_B0 = Source2_B0 = Source3_B0 = Field_Default;
   A1 = A2 = A3 = Field_Default;
   set_problem_config();
+  /* coordinate system of the seeded problem: spherical for the dipole
+     shell (kind 2), Cartesian for the others */
+  gk_coord_type = gk_problem_kind == 2 ? 2 : 0;
   if (initialize_config(0, 1, config_file) != 0) return -1;
   if (setup_comm_and_types(0) != 0) return -1;
   if (initialize_logging(BASE_LOG_NAME, 0) != 0) return -1;

@@ -21,6 +21,11 @@ CASES = [
     (go.SINUSOID_BLAST, 3, 16, 16, 16, 20),
     (go.DIPOLE_SHELL, 4, 16, 12, 16, 20),
     (go.SINUSOID_SPHERE, 5, 24, 16, 8, 40),
+    # 65 k faces: one 64-face tile of the staged k sweep plus a one-face
+    # remainder through the direct sweep; rows of the marching kernels span
+    # two lanes' rows within a wave
+    (go.SINUSOID, 6, 8, 12, 64, 10),
+    (go.DIPOLE_SHELL, 7, 12, 8, 70, 10),
 ]
 
 
