@@ -42,8 +42,16 @@ def kernel_alg_bytes(name, n):
         # B (1) in; 9 fluxes + 2 face velocities out.  The transverse
         # extension (8 extra rows) only forms the 2 face velocities: 2+2 in,
         # 2 out.
-        ext_faces = (n + 1) * (N - 1 - n) * n
-        return 8 * (full_faces * 30 + ext_faces * 6)
+        nf = n + 1
+        if name == "k_sweep_k":
+            # whole 64-face tiles; a remainder under 32 faces per row runs in
+            # the direct sweep (k_krem_k)
+            rem = nf % 64
+            if nf >= 64 and 0 < rem < 32:
+                nf -= rem
+        full = nf * n * n
+        ext_faces = nf * (N - 1 - n) * n
+        return 8 * (full * 30 + ext_faces * 6)
     if name.startswith("k_emf_"):
         # per edge: 2 face velocities, 2 face-B (n, n-1 each), density at n, n-1
         # in, 1 EMF out

@@ -642,80 +642,100 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
   }
 }
 
-// k sweep (D = 2) with the stencil staged in LDS: a block covers 64 faces
-// along k times KROWS rows along j at one i (one wave per block: no wave waits
-// at the barrier for another); each of the reconstructed fields
-// is loaded once per block (72 cells per row: 64 faces + 8 stencil halo) as
-// half-step and current values.  Planes i outside [flo0, fhi0] only form the
-// face-averaged velocities (v3, v1).
-constexpr int KROWS = 1;
+// k sweep (D = 2) with the stencil staged in LDS.  A block is one wave
+// covering 64 faces along k of one row (i, j); each reconstructed field is
+// staged once (72 cells: 64 faces + 8 stencil halo) as half-step and current
+// values.  All of the row's global loads are issued before the first LDS
+// write, so the wave waits for memory once, not once per field.  Planes i
+// outside [flo0, fhi0] only form the face-averaged velocities (v3, v1).
 constexpr int KSPAN = 72;
 constexpr int CELL_CHUNK = 1;  // cells each thread of the cell update walks along i
 
 template <typename R>
 __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, const int by,
                                          const int bz) {
-  __shared__ R sh[9][KROWS][KSPAN];
-  __shared__ R sx[9][KROWS][KSPAN];
+  __shared__ R sh[9][KSPAN];
+  __shared__ R sx[9][KSPAN];
   const Grid &g = a.g;
   const int k0 = a.lo[2] + bx * 64;
-  const int j0 = a.lo[1] + by * KROWS;
+  const int j = a.lo[1] + by;
   const int i = a.lo[0] + bz;
   const long fs = g.fs;
   const bool full = i >= a.flo[0] && i <= a.fhi[0];
   const R cc = (R)a.sc->c;
   R *F = (R *)a.F, *V = (R *)a.V;
-  // stage: points k0-4 .. k0+67 of rows j0 .. j0+KROWS-1 (clamped to the box);
-  // lane tx loads point tx and, for tx < 8, point 64 + tx of its row
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const long rowb = i * g.st[0] + min(j0 + ty, g.N[1] - 1) * g.st[1];
+  // lane tx stages point k0-4+tx and, for tx < 8, point k0+60+tx of the row
+  // (clamped to the box)
+  const int tx = threadIdx.x;
+  const bool halo = tx < KSPAN - 64;
+  const long rowb = i * g.st[0] + j * g.st[1];
   const long oa = rowb + min(k0 - 4 + tx, g.N[2] - 1);
   const long ob = rowb + min(k0 + 60 + tx, g.N[2] - 1);
-  auto stage = [&](int m, const R *P, const R *Q) {
-    const R xa = (R)P[oa], pa = (R)Q[oa];
-    sh[m][ty][tx] = half_step(xa, pa, cc);
-    sx[m][ty][tx] = xa;
-    if (tx < KSPAN - 64) {
-      const R xb = (R)P[ob], pb = (R)Q[ob];
-      sh[m][ty][64 + tx] = half_step(xb, pb, cc);
-      sx[m][ty][64 + tx] = xb;
-    }
-  };
+  // staged fields: all nine on full planes, else v3 (slot 3) and v1 (slot 1)
   const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
   const R *M = (const R *)a.M, *Mp = (const R *)a.Mp;
+  const R *P[9], *Q[9];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    P[m] = X + m * fs;
+    Q[m] = Xp + m * fs;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    P[6 + m] = M + m * fs;
+    Q[6 + m] = Mp + m * fs;
+  }
+  R xa[9], pa[9], xb[9], pb[9];
+  auto fetch = [&](int m) {
+    xa[m] = P[m][oa];
+    pa[m] = Q[m][oa];
+    if (halo) {
+      xb[m] = P[m][ob];
+      pb[m] = Q[m][ob];
+    }
+  };
+  auto stage = [&](int m) {
+    sh[m][tx] = half_step(xa[m], pa[m], cc);
+    sx[m][tx] = xa[m];
+    if (halo) {
+      sh[m][64 + tx] = half_step(xb[m], pb[m], cc);
+      sx[m][64 + tx] = xb[m];
+    }
+  };
   if (full) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) stage(m, X + m * fs, Xp + m * fs);
+    for (int m = 0; m < 9; m++) fetch(m);
 #pragma unroll
-    for (int m = 0; m < 3; m++) stage(6 + m, M + m * fs, Mp + m * fs);
+    for (int m = 0; m < 9; m++) stage(m);
   } else {
-    stage(3, X + 3 * fs, Xp + 3 * fs);
-    stage(1, X + 1 * fs, Xp + 1 * fs);
+    fetch(3);
+    fetch(1);
+    stage(3);
+    stage(1);
   }
   __syncthreads();
-  const int k = k0 + threadIdx.x;
-  const int j = j0 + threadIdx.y;
-  if (k > a.hi[2] || j > a.hi[1]) return;
+  const int k = k0 + tx;
+  if (k > a.hi[2]) return;
   int idx[3] = {i, j, k};
   const R *T = rtab<2, R>(a.geo, k, idx);
-  const long c = i * g.st[0] + j * g.st[1] + k;
-  const int ry = threadIdx.y, b = threadIdx.x;  // stencil cells b .. b+7 in LDS
+  const long c = rowb + k;
+  const int b = tx;  // stencil cells b .. b+7 in LDS
   auto lr = [&](int m, R &L, R &Rr) {
     R fl = R(0), fr = R(0);
 #pragma unroll
     for (int n = 0; n < 7; n++) {
-      fl = madd(sh[m][ry][b + n], T[n], fl);
-      fr = madd(sh[m][ry][b + 7 - n], T[7 + n], fr);
+      fl = madd(sh[m][b + n], T[n], fl);
+      fr = madd(sh[m][b + 7 - n], T[7 + n], fr);
     }
-    L = pdm_limit(fl, sx[m][ry][b + 2], sx[m][ry][b + 3], sx[m][ry][b + 4], T[14], T[15]);
-    Rr = pdm_limit(fr, sx[m][ry][b + 5], sx[m][ry][b + 4], sx[m][ry][b + 3], T[16], T[17]);
+    L = pdm_limit(fl, sx[m][b + 2], sx[m][b + 3], sx[m][b + 4], T[14], T[15]);
+    Rr = pdm_limit(fr, sx[m][b + 5], sx[m][b + 4], sx[m][b + 3], T[16], T[17]);
   };
   if (!full) {
-    R l, r;
-    lr(3, l, r);
-    V[c] = R(0.5) * (l + r);
-    lr(1, l, r);
-    V[fs + c] = R(0.5) * (l + r);
+    R l, rr;
+    lr(3, l, rr);
+    V[c] = R(0.5) * (l + rr);
+    lr(1, l, rr);
+    V[fs + c] = R(0.5) * (l + rr);
     return;
   }
   R L[6], Rs[6], bl[3], br[3];
@@ -2075,8 +2095,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       const bool split = wk >= 64 && rem > 0 && rem < 32;
       if (split) m.hi[2] = a.hi[2] - rem;
       L.kid = SK_SWEEP_K;
-      L.grid = dim3((m.hi[2] - m.lo[2] + 64) / 64, (m.hi[1] - m.lo[1] + KROWS) / KROWS,
-                    m.hi[0] - m.lo[0] + 1);
+      L.grid = dim3((m.hi[2] - m.lo[2] + 64) / 64, m.hi[1] - m.lo[1] + 1, m.hi[0] - m.lo[0] + 1);
       L.u.sw = m;
       out.push_back(L);
       if (split) {
@@ -2193,7 +2212,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_SWEEP_I: k_march<0, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64, KROWS), 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_KREM: k_sweep<2, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_EMF_I: k_emf_march<0, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_EMF_J: k_emf_march<1, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
@@ -2214,7 +2233,7 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
     case SK_SWEEP_I: k_march_batch<0, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64, KROWS), 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_KREM: k_sweep_batch<2, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_EMF_I: k_emf_march_batch<0, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_EMF_J: k_emf_march_batch<1, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
