@@ -994,10 +994,10 @@ template <typename R>
 __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const int by,
                                      const int bz) {
   const Grid &g = a.g;
-  const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
-  const int j = g.s[1] + by * 4 + threadIdx.y;
+  (void)by;
+  int k, j;
+  if (!row_point(g.s[2], g.e[2] + 1, g.s[1], g.e[1] + 1, bx, k, j)) return;
   const int i = g.s[0] + bz;
-  if (k > g.e[2] + 1 || j > g.e[1] + 1) return;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const long SI = g.st[0], SJ = g.st[1];
   const double dt = a.sc->dt;
@@ -1073,13 +1073,12 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
                                        const int bz) {
   const Grid &g = a.g;
   const Geo &G = a.geo;
-  const int k = g.s[2] + bx * 64 + threadIdx.x;
-  const int j = g.s[1] + by * 4 + threadIdx.y;
+  int k, j;
+  const bool valid = row_point(g.s[2], g.e[2], g.s[1], g.e[1], bx, k, j);
   // each thread walks CELL_CHUNK cells along i, so the i-face values a cell
   // shares with the next (fluxes, face B) are re-read from cache, not HBM
   const int i0 = g.s[0] + bz * CELL_CHUNK;
   const int i1 = min(i0 + CELL_CHUNK, g.e[0] + 1);
-  const bool valid = k <= g.e[2] && j <= g.e[1];
   double dtc = INFINITY, divb = 0.0;
   bool bad = false;
   if (valid)
@@ -1090,6 +1089,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     const double dt = a.sc->dt;
     const double CA = g.CA;
     const double V = geo3(G, GK_G_VOL, i, j, k);
+    const double iV = 1.0 / V;
     double A0[3], A1[3];
     A0[0] = geo3(G, GK_G_FACE_I, i, j, k);
     A1[0] = geo3(G, GK_G_FACE_I, i + 1, j, k);
@@ -1107,7 +1107,10 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
       const long sd = g.st[d];
 #pragma unroll
       for (int m = 0; m < 9; m++) {
-        double v = ((double)Fd[m * g.fs + c + sd] * A1[d] - (double)Fd[m * g.fs + c] * A0[d]) / V;
+        double v = (double)Fd[m * g.fs + c + sd] * A1[d] - (double)Fd[m * g.fs + c] * A0[d];
+        // FP64 divides every term by V as the reference does; FP32-bulk
+        // (fluxes already rounded to FP32) multiplies by 1/V
+        v = sizeof(R) == 4 ? v * iV : v / V;
         dU[m] = d == 0 ? v : dU[m] + v;
       }
     }
@@ -1333,15 +1336,51 @@ __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o
   return x;
 }
 
+// The points of the box outside the active cells, walked as three slabs:
+// i outside [s0, e0] (all j, k), then j outside [s1, e1] (i active, all k),
+// then k outside [s2, e2] (i, j active); k fastest in each, 256 per block.
+__device__ __forceinline__ bool ghost_point(const Grid &g, int bx, int &i, int &j, int &k) {
+  const int n0 = g.e[0] - g.s[0] + 1, n1 = g.e[1] - g.s[1] + 1, n2 = g.e[2] - g.s[2] + 1;
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  long p = (long)bx * 256 + threadIdx.y * 64 + threadIdx.x;
+  const long ca = (long)(N0 - n0) * N1 * N2, cb = (long)n0 * (N1 - n1) * N2;
+  const long cc = (long)n0 * n1 * (N2 - n2);
+  if (p < ca) {
+    k = (int)(p % N2);
+    const long t = p / N2;
+    j = (int)(t % N1);
+    const int q = (int)(t / N1);
+    i = q < g.s[0] ? q : q + n0;
+    return true;
+  }
+  p -= ca;
+  if (p < cb) {
+    k = (int)(p % N2);
+    const long t = p / N2;
+    const int q = (int)(t % (N1 - n1));
+    i = g.s[0] + (int)(t / (N1 - n1));
+    j = q < g.s[1] ? q : q + n1;
+    return true;
+  }
+  p -= cb;
+  if (p < cc) {
+    const int q = (int)(p % (N2 - n2));
+    const long t = p / (N2 - n2);
+    j = g.s[1] + (int)(t % n1);
+    i = g.s[0] + (int)(t / n1);
+    k = q < g.s[2] ? q : q + n2;
+    return true;
+  }
+  return false;
+}
+
 __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, const int by,
                                         const int bz) {
+  (void)by;
+  (void)bz;
   const Grid &g = a.g;
-  const int k = bx * blockDim.x + threadIdx.x;
-  const int j = by * 4 + threadIdx.y;
-  const int i = bz;
-  if (k >= g.N[2] || j >= g.N[1]) return;
-  if (i >= g.s[0] && i <= g.e[0] && j >= g.s[1] && j <= g.e[1] && k >= g.s[2] && k <= g.e[2])
-    return;
+  int i, j, k;
+  if (!ghost_point(g, bx, i, j, k)) return;
   const int x[3] = {i, j, k};
   const long c = i * g.st[0] + j * g.st[1] + k;
   {
@@ -1405,10 +1444,10 @@ struct MagsArgs {
 __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
                                        const int bz) {
   const Grid &g = a.g;
-  const int k = bx * blockDim.x + threadIdx.x;
-  const int j = by * 4 + threadIdx.y;
+  (void)by;
+  int k, j;
+  if (!row_point(0, g.N[2] - 2, 0, g.N[1] - 2, bx, k, j)) return;
   const int i = bz;
-  if (k > g.N[2] - 2 || j > g.N[1] - 2) return;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int idx[3] = {i, j, k};
 #pragma unroll
@@ -1773,6 +1812,26 @@ static dim3 box_grid(int nk, int nj, int ni) {
   return dim3((nk + 63) / 64, (nj + 3) / 4, ni);
 }
 
+// blocks of 256 points over the (row, k) plane of a launch
+static int row_blocks(int klo, int khi, int rlo, int rhi) {
+  return ((khi - klo + 1) * (rhi - rlo + 1) + 255) / 256;
+}
+
+// ghost fill: 256 points per block over the points outside the active cells
+static dim3 ghost_grid(const Grid &g) {
+  long n = 1, t = 1;
+  for (int d = 0; d < 3; d++) {
+    n *= g.e[d] - g.s[d] + 1;
+    t *= g.N[d];
+  }
+  return dim3((unsigned)((t - n + 255) / 256), 1, 1);
+}
+
+// cell-centred B over the box minus the upper face slot
+static dim3 mags_grid(const Grid &g) {
+  return dim3(row_blocks(0, g.N[2] - 2, 0, g.N[1] - 2), 1, g.N[0] - 1);
+}
+
 static void launch_ghost(gk_solver *s, double *X, double *B) {
   GhostArgs a;
   a.g = s->g;
@@ -1781,7 +1840,7 @@ static void launch_ghost(gk_solver *s, double *X, double *B) {
   a.X32 = nullptr;
   a.B32 = nullptr;
   memcpy(a.bc, s->bc, sizeof(a.bc));
-  const dim3 gr = box_grid(s->g.N[2], s->g.N[1], s->g.N[0]);
+  const dim3 gr = ghost_grid(s->g);
   const int tot = gr.x * gr.y * gr.z;
   k_ghost<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
 }
@@ -1793,7 +1852,7 @@ static void launch_mags(gk_solver *s, const double *B, double *M) {
   a.B = B;
   a.M = M;
   a.M32 = nullptr;
-  const dim3 gr = box_grid(s->g.N[2] - 1, s->g.N[1] - 1, s->g.N[0] - 1);
+  const dim3 gr = mags_grid(s->g);
   const int tot = gr.x * gr.y * gr.z;
   k_mags<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
 }
@@ -1996,11 +2055,6 @@ struct Rot {
 static Rot rot_of(const gk_solver *s) { return Rot{s->cur, s->b_m1, s->b_0, s->b_n}; }
 static Rot rot_next(Rot r) { return Rot{1 - r.cur, r.b_0, r.b_n, r.b_m1}; }
 
-// blocks of 256 points over the (row, k) plane of a marching launch
-static int row_blocks(int klo, int khi, int rlo, int rhi) {
-  return ((khi - klo + 1) * (rhi - rlo + 1) + 255) / 256;
-}
-
 // blocks of 256 points over the launch box of the direct sweep
 static int point_blocks(const SweepArgs &v) {
   const long n = (long)(v.hi[2] - v.lo[2] + 1) * (v.hi[1] - v.lo[1] + 1) * (v.hi[0] - v.lo[0] + 1);
@@ -2147,7 +2201,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.E = s->E;
     a.sc = s->sc;
     L.kid = SK_CT;
-    L.grid = box_grid(s->nk + 1, s->nj + 1, s->ni + 1);
+    L.grid = dim3(row_blocks(S[2], Ee[2] + 1, S[1], Ee[1] + 1), 1, s->ni + 1);
     L.u.ct = a;
     out.push_back(L);
   }
@@ -2168,7 +2222,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
     L.kid = SK_CELL;
-    L.grid = dim3((s->nk + 63) / 64, (s->nj + 3) / 4, (s->ni + CELL_CHUNK - 1) / CELL_CHUNK);
+    L.grid = dim3(row_blocks(S[2], Ee[2], S[1], Ee[1]), 1, (s->ni + CELL_CHUNK - 1) / CELL_CHUNK);
     L.u.ce = a;
     out.push_back(L);
   }
@@ -2182,7 +2236,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.B32 = f32 ? s->B32[r.b_n] : nullptr;
     memcpy(a.bc, s->bc, sizeof(a.bc));
     L.kid = SK_GHOST;
-    L.grid = box_grid(g.N[2], g.N[1], g.N[0]);
+    L.grid = ghost_grid(g);
     L.u.gh = a;
     out.push_back(L);
   }
@@ -2195,7 +2249,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.M = Mn;
     a.M32 = f32 ? s->M32[r.b_n] : nullptr;
     L.kid = SK_MAGS;
-    L.grid = box_grid(g.N[2] - 1, g.N[1] - 1, g.N[0] - 1);
+    L.grid = mags_grid(g);
     L.u.mg = a;
     out.push_back(L);
   }
