@@ -855,6 +855,55 @@ __device__ __forceinline__ void shift1(Window1<R> &w) {
   for (int n = 0; n < 7; n++) w.h[n] = w.h[n + 1];
 }
 
+// Reconstruction table of axis A at a fixed face for a point that marches
+// along MA, one of A's transverse axes: only the class of the march
+// coordinate changes from one position to the next.  That class is looked up
+// one position ahead, so the table loads of a position are issued together
+// with its field loads instead of waiting for the lookup.
+template <int A, int MA, typename R>
+struct TableWalk {
+  const R *base;
+  long mul;
+  const int *cls;
+  int next;
+  __device__ __forceinline__ void init(const Geo &g, int face, const int idx[3], int f0) {
+    constexpr int B = A == 0 ? 1 : 0;
+    constexpr int C = A == 2 ? 1 : 2;
+    static_assert(MA == B || MA == C, "march axis must be transverse to the table axis");
+    const R *t;
+    if constexpr (sizeof(R) == 8)
+      t = (const R *)g.recw[A];
+    else
+      t = (const R *)g.recwf[A];
+    const long n0 = g.ncls[A][0], n1 = g.ncls[A][1];
+    if (MA == B) {  // fixed class on C
+      base = t + ((long)face * n0 * n1 + g.cls[A][1][idx[C]]) * GK_RECW_STRIDE;
+      mul = n1 * GK_RECW_STRIDE;
+      cls = g.cls[A][0];
+    } else {        // fixed class on B
+      base = t + ((long)face * n0 + g.cls[A][0][idx[B]]) * n1 * GK_RECW_STRIDE;
+      mul = GK_RECW_STRIDE;
+      cls = g.cls[A][1];
+    }
+    next = cls[f0];
+  }
+  // table at march position f (positions are visited f0, f0 + 1, ... < f1)
+  __device__ __forceinline__ const R *at(int f, int f1) {
+    const R *t = base + next * mul;
+    if (f + 1 < f1) next = cls[f + 1];
+    return t;
+  }
+};
+
+// Occupancy bound of the edge-field kernels.  The FP64 E_j and E_k marches
+// keep three windows and reconstruct a fourth field directly; at up to two
+// waves per SIMD they have the registers to issue a position's table and
+// field loads together (at three the compiler loads the table piecewise).
+template <int D, typename R>
+struct EmfWaves {
+  static constexpr int hi = (sizeof(R) == 8 && D != 0) ? 2 : 8;
+};
+
 template <int D, typename R>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
@@ -907,6 +956,10 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
     }
     march_load<1, R, R>(wb, Pw, Qw, o, cc, p);
   }
+  // direct reconstructions: v and F1 along A2 (E_i), else F2 along R2
+  constexpr int DA = VW ? R2 : A2;
+  TableWalk<DA, MA, R> tw;
+  tw.init(a.geo, idx[DA], idx, f0);
   // densities at march position f0 - 1 (offsets -1 and 0 along O)
   R u0 = rh(c - sm - so), u1 = rh(c - sm);
   for (int f = f0; f < f1; f++) {
@@ -916,7 +969,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       wv1.h[7] = V1[o];
     }
     march_load<1, R, R>(wb, Pw, Qw, o, cc, 7);
-    idx[MA] = f;
+    const R *Td = tw.at(f, f1);
     R va, vb, b1l, b1r, b2l, b2r, l, rr;
     if (VW) {
       lr_self(wv0, Tm, l, rr);
@@ -924,9 +977,9 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       lr_self(wv1, Tm, l, rr);
       vb = R(0.5) * (l + rr);
       march_lr(wb, 0, Tm, b1l, b1r);
-      recon(B2, B2p, c, g.st[R2], cc, rtab<R2, R>(a.geo, idx[R2], idx), b2l, b2r);
+      recon(B2, B2p, c, g.st[R2], cc, Td, b2l, b2r);
     } else {
-      const R *T2 = rtab<A2, R>(a.geo, idx[A2], idx);
+      const R *T2 = Td;
       recon(V0, (const R *)nullptr, c, g.st[A2], cc, T2, l, rr);
       va = R(0.5) * (l + rr);
       recon(V1, (const R *)nullptr, c, g.st[A2], cc, T2, l, rr);
@@ -955,7 +1008,8 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
 }
 
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
+__global__ __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
   k_emf_march_body<D, R>(a, bx, by, bz);
@@ -963,7 +1017,8 @@ __global__ __launch_bounds__(256) void k_emf_march(EmfArgs a, int gx, int gy, in
 
 // one launch for many grids: blocks walk a work list of (argument set, block)
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_emf_march_batch(const EmfArgs *__restrict__ args,
+__global__ __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march_batch(const EmfArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     // uniform per block: keep the argument set in scalar registers
