@@ -384,6 +384,27 @@ __device__ __forceinline__ bool xcd_block(int gx, int gy, int gz, int &bx, int &
   return true;
 }
 
+// XCD-aware order in 8 x 8 patches of (x, z) for launches with gy == 1
+// whose blocks read faces of both neighbours, x + 1 (the next row tile) and
+// z + 1 (the next plane): inside a patch the two neighbours of a block run
+// within 8 blocks of it on the same XCD, so the shared faces are still in
+// its L2.  The launch covers the grid rounded up to whole patches
+// (patch_blocks); blocks outside the grid return at once.
+constexpr int PATCH = 8;
+
+__device__ __forceinline__ bool xcd_patch(int gx, int gz, int &bx, int &bz) {
+  const int npz = (gz + PATCH - 1) / PATCH;
+  const int total = (gx + PATCH - 1) / PATCH * PATCH * npz * PATCH;
+  const int per = (total + NXCD - 1) / NXCD;
+  const int b = blockIdx.x;
+  const int L = (b % NXCD) * per + b / NXCD;
+  if (L >= total) return false;
+  const int pp = L / (PATCH * PATCH), w = L % (PATCH * PATCH);
+  bz = pp % npz * PATCH + w % PATCH;
+  bx = pp / npz * PATCH + w / PATCH;
+  return bx < gx && bz < gz;
+}
+
 // work-list entry of a batched launch: argument set and block coordinates
 struct WorkItem {
   int arg, bx, by, bz;
@@ -1085,7 +1106,9 @@ __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const i
 template <typename R>
 __global__ __launch_bounds__(256) void k_ct(CtArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
-  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  (void)gy;
+  by = 0;
+  if (!xcd_patch(gx, gz, bx, bz)) return;
   k_ct_body<R>(a, bx, by, bz);
 }
 
@@ -1302,7 +1325,9 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
 template <typename R>
 __global__ __launch_bounds__(256) void k_cell(CellArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
-  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  (void)gy;
+  by = 0;
+  if (!xcd_patch(gx, gz, bx, bz)) return;
   k_cell_body<R>(a, bx, by, bz);
 }
 
@@ -1536,7 +1561,9 @@ __global__ __launch_bounds__(256) void k_to32(const double *__restrict__ src,
 
 __global__ __launch_bounds__(256) void k_mags(MagsArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
-  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  (void)gy;
+  by = 0;
+  if (!xcd_patch(gx, gz, bx, bz)) return;
   k_mags_body(a, bx, by, bz);
 }
 
@@ -1882,6 +1909,11 @@ static dim3 ghost_grid(const Grid &g) {
   return dim3((unsigned)((t - n + 255) / 256), 1, 1);
 }
 
+// blocks a launch decoded by xcd_patch needs (whole 8 x 8 patches)
+static int patch_blocks(const dim3 &g) {
+  return (int)((g.x + PATCH - 1) / PATCH * PATCH * ((g.z + PATCH - 1) / PATCH) * PATCH);
+}
+
 // cell-centred B over the box minus the upper face slot
 static dim3 mags_grid(const Grid &g) {
   return dim3(row_blocks(0, g.N[2] - 2, 0, g.N[1] - 2), 1, g.N[0] - 1);
@@ -1908,7 +1940,7 @@ static void launch_mags(gk_solver *s, const double *B, double *M) {
   a.M = M;
   a.M32 = nullptr;
   const dim3 gr = mags_grid(s->g);
-  const int tot = gr.x * gr.y * gr.z;
+  const int tot = patch_blocks(gr);
   k_mags<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
 }
 
@@ -2314,8 +2346,10 @@ template <typename R>
 static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 blk(64, 4);
   const int gx = L.grid.x, gy = L.grid.y, gz = L.grid.z;
-  const int total = gx * gy * gz;
-  const dim3 g1((total + NXCD - 1) / NXCD * NXCD);  // decoded by xcd_block
+  // decoded by xcd_patch (cell update, CT, cell B) or xcd_block (the others)
+  const bool patch = L.kid == SK_CT || L.kid == SK_CELL || L.kid == SK_MAGS;
+  const int total = patch ? patch_blocks(L.grid) : gx * gy * gz;
+  const dim3 g1((total + NXCD - 1) / NXCD * NXCD);
   switch (L.kid) {
     case SK_BEGIN: k_begin<<<1, NSLOT, 0, st>>>(L.u.sc); break;
     case SK_SWEEP_I: k_march<0, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
