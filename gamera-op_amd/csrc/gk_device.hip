@@ -114,10 +114,13 @@ __device__ __forceinline__ double cfl_dt(double rho, double v1, double v2, doubl
 }
 
 // cell-centred B component m from the face field (src/solver_func.c:310-322)
+__device__ __forceinline__ double cell_b_of(double b_lo, double b_hi, const Geo &G, int n, int m) {
+  const double *d0 = G.cc[m][0], *d1 = G.cc[m][1], *dx = G.cc[m][2];
+  return b_lo * d0[n] / dx[n] + b_hi * d1[n] / dx[n];
+}
 __device__ __forceinline__ double cell_b(const double *Bm, const Geo &G, long c, long sm, int n,
                                          int m) {
-  const double *d0 = G.cc[m][0], *d1 = G.cc[m][1], *dx = G.cc[m][2];
-  return Bm[c] * d0[n] / dx[n] + Bm[c + sm] * d1[n] / dx[n];
+  return cell_b_of(Bm[c], Bm[c + sm], G, n, m);
 }
 
 // block-wide min / max of non-negative doubles through their bit images
@@ -1080,24 +1083,36 @@ __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const i
   const R *Ei = (const R *)a.E, *Ej = Ei + g.fs, *Ek = Ei + 2 * g.fs;
   const Geo &G = a.geo;
   const bool ai = i <= g.e[0], aj = j <= g.e[1], ak = k <= g.e[2];
+  // every input of the point (edge fields, face B, metric factors) is read
+  // before the first store, so the wave waits for memory once; neighbour
+  // indices stay inside the box for i, j, k <= e + 1
+  const double ei = (double)Ei[c], ei_j = (double)Ei[c + SJ], ei_k = (double)Ei[c + 1];
+  const double ej = (double)Ej[c], ej_i = (double)Ej[c + SI], ej_k = (double)Ej[c + 1];
+  const double ek = (double)Ek[c], ek_i = (double)Ek[c + SI], ek_j = (double)Ek[c + SJ];
+  const double bi = a.B[c], bj = a.B[g.fs + c], bk = a.B[2 * g.fs + c];
+  const double li = geo3(G, GK_G_EDGE_I, i, j, k), li_j = geo3(G, GK_G_EDGE_I, i, j + 1, k),
+               li_k = geo3(G, GK_G_EDGE_I, i, j, k + 1);
+  const double lj = geo3(G, GK_G_EDGE_J, i, j, k), lj_i = geo3(G, GK_G_EDGE_J, i + 1, j, k),
+               lj_k = geo3(G, GK_G_EDGE_J, i, j, k + 1);
+  const double lk = geo3(G, GK_G_EDGE_K, i, j, k), lk_i = geo3(G, GK_G_EDGE_K, i + 1, j, k),
+               lk_j = geo3(G, GK_G_EDGE_K, i, j + 1, k);
+  const double si = geo3(G, GK_G_FACE_I, i, j, k), sj = geo3(G, GK_G_FACE_J, i, j, k),
+               sk = geo3(G, GK_G_FACE_K, i, j, k);
   if (aj && ak) {  // B_i on faces i = is..ie+1
-    double d = ((double)Ek[c + SJ] * geo3(G, GK_G_EDGE_K, i, j + 1, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k)) -
-               ((double)Ej[c + 1] * geo3(G, GK_G_EDGE_J, i, j, k + 1) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k));
-    const double b = a.B[c] - dt * d / geo3(G, GK_G_FACE_I, i, j, k);
+    const double d = (ek_j * lk_j - ek * lk) - (ej_k * lj_k - ej * lj);
+    const double b = bi - dt * d / si;
     a.Bn[c] = b;
     if (a.Bn32) a.Bn32[c] = (float)b;
   }
   if (ai && ak) {  // B_j on faces j = js..je+1
-    double d = ((double)Ei[c + 1] * geo3(G, GK_G_EDGE_I, i, j, k + 1) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k)) -
-               ((double)Ek[c + SI] * geo3(G, GK_G_EDGE_K, i + 1, j, k) - (double)Ek[c] * geo3(G, GK_G_EDGE_K, i, j, k));
-    const double b = a.B[g.fs + c] - dt * d / geo3(G, GK_G_FACE_J, i, j, k);
+    const double d = (ei_k * li_k - ei * li) - (ek_i * lk_i - ek * lk);
+    const double b = bj - dt * d / sj;
     a.Bn[g.fs + c] = b;
     if (a.Bn32) a.Bn32[g.fs + c] = (float)b;
   }
   if (ai && aj) {  // B_k on faces k = ks..ke+1
-    double d = ((double)Ej[c + SI] * geo3(G, GK_G_EDGE_J, i + 1, j, k) - (double)Ej[c] * geo3(G, GK_G_EDGE_J, i, j, k)) -
-               ((double)Ei[c + SJ] * geo3(G, GK_G_EDGE_I, i, j + 1, k) - (double)Ei[c] * geo3(G, GK_G_EDGE_I, i, j, k));
-    const double b = a.B[2 * g.fs + c] - dt * d / geo3(G, GK_G_FACE_K, i, j, k);
+    const double d = (ej_i * lj_i - ej * lj) - (ei_j * li_j - ei * li);
+    const double b = bk - dt * d / sk;
     a.Bn[2 * g.fs + c] = b;
     if (a.Bn32) a.Bn32[2 * g.fs + c] = (float)b;
   }
@@ -1168,6 +1183,10 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     const double CA = g.CA;
     const double V = geo3(G, GK_G_VOL, i, j, k);
     const double iV = 1.0 / V;
+    // cell widths of the CFL dt, read here: loads after the state stores
+    // below could not be moved ahead of them
+    const double dxi = geo3(G, GK_G_DXI, i, j, k), dxj = geo3(G, GK_G_DXJ, i, j, k),
+                 dxk = geo3(G, GK_G_DXK, i, j, k);
     double A0[3], A1[3];
     A0[0] = geo3(G, GK_G_FACE_I, i, j, k);
     A1[0] = geo3(G, GK_G_FACE_I, i + 1, j, k);
@@ -1201,6 +1220,18 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     double bold[3];
 #pragma unroll
     for (int m = 0; m < 3; m++) bold[m] = a.M[m * g.fs + c];
+    // face B at n+1 on the cell's six faces (cell-centred B and div B below),
+    // loaded with the other inputs so the wave waits for memory once
+    double bf[3][2];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      bf[m][0] = a.Bn[m * g.fs + c];
+      bf[m][1] = a.Bn[m * g.fs + c + g.st[m]];
+    }
+    // cell B at n+1 (same formula as the reference)
+    double bnew[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) bnew[m] = cell_b_of(bf[m][0], bf[m][1], G, idx[m], m);
 
     double SH[3] = {0.0, 0.0, 0.0}, SB[3] = {0.0, 0.0, 0.0};
     if (a.spherical) {
@@ -1266,13 +1297,10 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     }
     if (p < P_FLOOR) p = P_FLOOR;
 
-    // cell B at n+1 (same formula as the reference), time-centred total field
-    double bnew[3], mt[3];
+    // time-centred total field
+    double mt[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) {
-      bnew[m] = cell_b(a.Bn + m * g.fs, G, c, g.st[m], idx[m], m);
-      mt[m] = 0.5 * (bnew[m] + bold[m]) + a.b0.cell[m];
-    }
+    for (int m = 0; m < 3; m++) mt[m] = 0.5 * (bnew[m] + bold[m]) + a.b0.cell[m];
     double alf = (mt[0] * mt[0] + mt[1] * mt[1] + mt[2] * mt[2]) / rhof / (CA * CA);
     double perp = 1 / (1 + alf);
     double dva = alf * (-dt * dU[0]);
@@ -1302,16 +1330,13 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     }
     // next step's CFL dt from the new state (magtot = cell B + B0)
     double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0],
-                          bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2],
-                          geo3(G, GK_G_DXI, i, j, k), geo3(G, GK_G_DXJ, i, j, k),
-                          geo3(G, GK_G_DXK, i, j, k), CA);
+                          bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2], dxi, dxj, dxk, CA);
     if (dcell == dcell) dtc = fmin(dtc, dcell);  // fmin in the reference ignores NaN
     // div B of the new face field (with the B0 face normals)
-    const double *Bf = a.Bn;
     const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
-    double db = ((Bf[c + g.st[0]] + f0[0]) * A1[0] - (Bf[c] + f0[0]) * A0[0]) / V +
-                ((Bf[g.fs + c + g.st[1]] + f1[1]) * A1[1] - (Bf[g.fs + c] + f1[1]) * A0[1]) / V +
-                ((Bf[2 * g.fs + c + 1] + f2[2]) * A1[2] - (Bf[2 * g.fs + c] + f2[2]) * A0[2]) / V;
+    double db = ((bf[0][1] + f0[0]) * A1[0] - (bf[0][0] + f0[0]) * A0[0]) / V +
+                ((bf[1][1] + f1[1]) * A1[1] - (bf[1][0] + f1[1]) * A0[1]) / V +
+                ((bf[2][1] + f2[2]) * A1[2] - (bf[2][0] + f2[2]) * A0[2]) / V;
     const double adb = fabs(db);
     if (adb == adb)
       divb = fmax(divb, adb);
