@@ -1488,30 +1488,51 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
   if (!ghost_point(g, bx, i, j, k)) return;
   const int x[3] = {i, j, k};
   const long c = i * g.st[0] + j * g.st[1] + k;
-  {
-    int m0 = gmap(i, g.s[0], g.e[0], a.bc[0][0], a.bc[0][1], 0);
-    int m1 = gmap(j, g.s[1], g.e[1], a.bc[0][2], a.bc[0][3], 0);
-    int m2 = gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0);
-    long src = m0 * g.st[0] + m1 * g.st[1] + m2;
-    if (src != c) {
-#pragma unroll
-      for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = a.X[m * g.fs + src];
-      if (a.X32) {
-#pragma unroll
-        for (int m = 0; m < 6; m++) a.X32[m * g.fs + c] = a.X32[m * g.fs + src];
-      }
-    }
-  }
+  // sources are active points, which this launch never writes, so every
+  // value is read before the first store (a load after a store could not be
+  // moved ahead of it)
+  const long sx = gmap(i, g.s[0], g.e[0], a.bc[0][0], a.bc[0][1], 0) * g.st[0] +
+                  gmap(j, g.s[1], g.e[1], a.bc[0][2], a.bc[0][3], 0) * g.st[1] +
+                  gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0);
+  long sb[3];
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     int m[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
       m[d] = gmap(x[d], g.s[d], g.e[d], a.bc[1][2 * d], a.bc[1][2 * d + 1], d == f ? 1 : 0);
-    long src = m[0] * g.st[0] + m[1] * g.st[1] + m[2];
-    if (src != c) {
-      a.B[f * g.fs + c] = a.B[f * g.fs + src];
-      if (a.B32) a.B32[f * g.fs + c] = a.B32[f * g.fs + src];
+    sb[f] = m[0] * g.st[0] + m[1] * g.st[1] + m[2];
+  }
+  double xv[6], bv[3];
+  float xw[6], bw[3];
+  if (sx != c) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) xv[m] = a.X[m * g.fs + sx];
+    if (a.X32) {
+#pragma unroll
+      for (int m = 0; m < 6; m++) xw[m] = a.X32[m * g.fs + sx];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    if (sb[f] != c) {
+      bv[f] = a.B[f * g.fs + sb[f]];
+      if (a.B32) bw[f] = a.B32[f * g.fs + sb[f]];
+    }
+  }
+  if (sx != c) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = xv[m];
+    if (a.X32) {
+#pragma unroll
+      for (int m = 0; m < 6; m++) a.X32[m * g.fs + c] = xw[m];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    if (sb[f] != c) {
+      a.B[f * g.fs + c] = bv[f];
+      if (a.B32) a.B32[f * g.fs + c] = bw[f];
     }
   }
 }
@@ -1555,11 +1576,15 @@ __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, con
   const int i = bz;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int idx[3] = {i, j, k};
+  // all three components before the first store (a load after a store
+  // could not be moved ahead of it)
+  double b[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) b[m] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
-    const double b = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
-    a.M[m * g.fs + c] = b;
-    if (a.M32) a.M32[m * g.fs + c] = (float)b;
+    a.M[m * g.fs + c] = b[m];
+    if (a.M32) a.M32[m * g.fs + c] = (float)b[m];
   }
 }
 
