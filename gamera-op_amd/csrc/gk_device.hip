@@ -408,7 +408,16 @@ __device__ __forceinline__ bool xcd_patch(int gx, int gz, int &bx, int &bz) {
   return bx < gx && bz < gz;
 }
 
-// work-list entry of a batched launch: argument set and block coordinates
+// work-list entry of a batched launch: argument set and block coordinates.
+// A batched launch has one block per entry; consecutive entries (neighbouring
+// blocks of one grid) go to the same XCD, as in xcd_block.
+__device__ __forceinline__ bool xcd_item(int nwork, int &w) {
+  const int per = (nwork + NXCD - 1) / NXCD;
+  const int b = blockIdx.x;
+  w = (b % NXCD) * per + b / NXCD;
+  return w < nwork;
+}
+
 struct WorkItem {
   int arg, bx, by, bz;
 };
@@ -490,19 +499,18 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, int gx, int gy, int 
   k_sweep_body<D, R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <int D, typename R>
 __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_sweep_body<D, R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_sweep_body<D, R>(args[arg], bx, by, bz);
 }
 
 // Marching sweep along D = 0 (i) or 1 (j).  Lanes run along k (coalesced),
@@ -651,19 +659,18 @@ __global__ __launch_bounds__(256) void k_march(SweepArgs a, int gx, int gy, int 
   k_march_body<D, R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <int D, typename R>
 __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_march_body<D, R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_march_body<D, R>(args[arg], bx, by, bz);
 }
 
 // k sweep (D = 2) with the stencil staged in LDS.  A block is one wave
@@ -782,19 +789,18 @@ __global__ __launch_bounds__(256) void k_sweepk(SweepArgs a, int gx, int gy, int
   k_sweepk_body<R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <typename R>
 __global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_sweepk_body<R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_sweepk_body<R>(args[arg], bx, by, bz);
 }
 
 // Edge EMF from the edge velocities (va, vb), the two reconstructed face-B
@@ -1039,20 +1045,19 @@ __global__ __launch_bounds__(256)
   k_emf_march_body<D, R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <int D, typename R>
 __global__ __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march_batch(const EmfArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_emf_march_body<D, R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_emf_march_body<D, R>(args[arg], bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1127,19 +1132,18 @@ __global__ __launch_bounds__(256) void k_ct(CtArgs a, int gx, int gy, int gz) {
   k_ct_body<R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <typename R>
 __global__ __launch_bounds__(256) void k_ct_batch(const CtArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_ct_body<R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_ct_body<R>(args[arg], bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1356,19 +1360,18 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a, int gx, int gy, int gz
   k_cell_body<R>(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 template <typename R>
 __global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_cell_body<R>(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_cell_body<R>(args[arg], bx, by, bz);
 }
 
 // CFL dt and |div B| of the current state (prologue; src/solver_func.c:255-343)
@@ -1543,18 +1546,17 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a, int gx, int gy, int 
   k_ghost_body(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 __global__ __launch_bounds__(256) void k_ghost_batch(const GhostArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_ghost_body(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_ghost_body(args[arg], bx, by, bz);
 }
 
 // cell-centred B over cells 0..N-2 of each axis (get_derived_variables,
@@ -1617,18 +1619,17 @@ __global__ __launch_bounds__(256) void k_mags(MagsArgs a, int gx, int gy, int gz
   k_mags_body(a, bx, by, bz);
 }
 
-// one launch for many grids: blocks walk a work list of (argument set, block)
+// one launch for many grids: block w runs work-list entry w (argument set, block)
 __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    // uniform per block: keep the argument set in scalar registers
-    const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-    const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
-    const int by = __builtin_amdgcn_readfirstlane(work[w].by);
-    const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-    k_mags_body(args[arg], bx, by, bz);
-    __syncthreads();
-  }
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  // uniform per block: keep the argument set in scalar registers
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_mags_body(args[arg], bx, by, bz);
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -2730,7 +2731,7 @@ static void batch_launch_step(gk_batch *b, int p, bool timed) {
     for (int gp = 0; gp < 2; gp++) {
       const int nw = b->nwork[gp][k];
       if (!nw) continue;
-      const int nblocks = nw < 4096 ? nw : 4096;
+      const int nblocks = (nw + NXCD - 1) / NXCD * NXCD;  // one block per entry (xcd_item)
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (timed) {
         (void)hipEventCreate(&e0);

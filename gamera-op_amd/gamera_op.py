@@ -370,6 +370,18 @@ class Solver:
     def device_bytes(self):
         return int(self.op.gk_device_bytes(self.handle))
 
+    def enable_timing(self, on=True):
+        _check(self.op.gk_enable_timing(self.handle, 1 if on else 0), "gk_enable_timing")
+
+    def kernel_times(self):
+        """Average ms per launch of each stage kernel over the last step() call
+        (with timing enabled)."""
+        n = ctypes.c_int(32)
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_double * 32)()
+        _check(self.op.gk_kernel_times(self.handle, names, ms, ctypes.byref(n)), "gk_kernel_times")
+        return {names[q].decode(): ms[q] for q in range(min(n.value, 32))}
+
     def autotune(self, trial_steps=3):
         best = ctypes.c_int(0)
         _check(self.op.gk_autotune(self.handle, trial_steps, ctypes.byref(best)), "gk_autotune")
