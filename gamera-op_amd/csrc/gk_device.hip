@@ -1424,8 +1424,8 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
 // point another thread writes.
 struct GhostArgs {
   Grid g;
-  double *X;    // 6 cell fields
-  double *B;    // 3 face fields
+  double *X;    // 6 cell fields, or null (not filled)
+  double *B;    // 3 face fields, or null (not filled)
   float *X32;   // FP32 copies (FP32-bulk form), else null
   float *B32;
   int bc[2][6];
@@ -1509,8 +1509,10 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
   double xv[6], bv[3];
   float xw[6], bw[3];
   if (sx != c) {
+    if (a.X) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) xv[m] = a.X[m * g.fs + sx];
+      for (int m = 0; m < 6; m++) xv[m] = a.X[m * g.fs + sx];
+    }
     if (a.X32) {
 #pragma unroll
       for (int m = 0; m < 6; m++) xw[m] = a.X32[m * g.fs + sx];
@@ -1519,13 +1521,15 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     if (sb[f] != c) {
-      bv[f] = a.B[f * g.fs + sb[f]];
+      if (a.B) bv[f] = a.B[f * g.fs + sb[f]];
       if (a.B32) bw[f] = a.B32[f * g.fs + sb[f]];
     }
   }
   if (sx != c) {
+    if (a.X) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = xv[m];
+      for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = xv[m];
+    }
     if (a.X32) {
 #pragma unroll
       for (int m = 0; m < 6; m++) a.X32[m * g.fs + c] = xw[m];
@@ -1534,7 +1538,7 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     if (sb[f] != c) {
-      a.B[f * g.fs + c] = bv[f];
+      if (a.B) a.B[f * g.fs + c] = bv[f];
       if (a.B32) a.B32[f * g.fs + c] = bw[f];
     }
   }
@@ -1995,9 +1999,18 @@ static void launch_mags(gk_solver *s, const double *B, double *M) {
   k_mags<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
 }
 
+// FP32-bulk form: the steps leave the ghosts of the FP64 primitives unfilled
+// (only the FP32 copies' ghosts are read); fill both time levels before the
+// FP64 state is read back or copied to FP32
+static void fill_x64_ghosts(gk_solver *s) {
+  if (s->precision != 1) return;
+  for (int b = 0; b < 2; b++) launch_ghost(s, s->X[b], nullptr);
+}
+
 // FP32 copies of both time levels (FP32-bulk form), after the state is set
 static void refresh_copies32(gk_solver *s) {
   if (s->precision != 1) return;
+  fill_x64_ghosts(s);
   const size_t fs = s->fs;
   const dim3 grid(2048), blk(256);
   for (int b = 0; b < 2; b++) hipLaunchKernelGGL(k_to32, grid, blk, 0, s->stream, s->X[b], s->X32[b], 6 * fs);
@@ -2368,7 +2381,10 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     GhostArgs a;
     memset(&a, 0, sizeof a);
     a.g = g;
-    a.X = Xn;
+    // the FP32-bulk stencils read the ghosts of X32; the FP64 state is only
+    // read at active cells during a step, so its ghosts are filled when the
+    // state is read back (fill_x64_ghosts)
+    a.X = f32 ? nullptr : Xn;
     a.B = Bn;
     a.X32 = f32 ? s->X32[1 - r.cur] : nullptr;
     a.B32 = f32 ? s->B32[r.b_n] : nullptr;
@@ -2483,6 +2499,7 @@ extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], dou
   if (!s) return GK_ERR_ARG;
   const size_t fs = s->fs;
   int rc;
+  if (prim || prim_prev) fill_x64_ghosts(s);
   for (int m = 0; m < 6; m++) {
     if (prim && prim[m] && (rc = copy_out(s, prim[m], s->X[s->cur] + m * fs)) != GK_OK) return rc;
     if (prim_prev && prim_prev[m] &&
