@@ -520,6 +520,116 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
 // limiter need: each face loads one new cell instead of eight.  Rows outside
 // [flo, fhi] of RA only form the two face-averaged velocities.  The per-face
 // arithmetic is that of k_sweep.
+// k faces of a few k columns (those left over after the 64-face tiles of
+// k_sweepk): a block is one wave on one column and 64 rows (j fastest, then
+// i).  A lane's stencil lies along k inside its own row, so each field is
+// loaded with 8 lanes per row (8 rows x 64 contiguous bytes per load) and
+// transposed through LDS, instead of every lane loading its own 8 cells (64
+// rows per load).  Rows of planes outside [flo0, fhi0] only store the face
+// velocities.  The arithmetic is that of k_sweepk.
+template <typename R>
+__device__ __forceinline__ void k_kcol_body(const SweepArgs &a, const int bx, const int by,
+                                       const int bz) {
+  (void)by;
+  (void)bz;
+  __shared__ R sx[64][9], sh[64][9];  // [row][cell], padded against bank conflicts
+  const Grid &g = a.g;
+  const int W = a.hi[2] - a.lo[2] + 1, H = a.hi[1] - a.lo[1] + 1;
+  const int nrow = H * (a.hi[0] - a.lo[0] + 1);
+  const int col = bx % W, grp = bx / W;
+  const int k = a.lo[2] + col;
+  const int tx = threadIdx.x;
+  const long fs = g.fs;
+  const R cc = (R)a.sc->c;
+  auto rowbase = [&](int q) {
+    q = min(q, nrow - 1);
+    return (long)(a.lo[0] + q / H) * g.st[0] + (long)(a.lo[1] + q % H) * g.st[1];
+  };
+  // loads: lane tx brings cell k-4+(tx%8) of row 8s + tx/8, s = 0..7
+  long lo_off[8];
+#pragma unroll
+  for (int s2 = 0; s2 < 8; s2++) lo_off[s2] = rowbase(grp * 64 + 8 * s2 + tx / 8) + k - 4 + tx % 8;
+  const R *X = (const R *)a.X, *Xp = (const R *)a.Xp;
+  const R *M = (const R *)a.M, *Mp = (const R *)a.Mp;
+  const int q = grp * 64 + tx;
+  const bool valid = q < nrow;
+  const int i = a.lo[0] + min(q, nrow - 1) / H, j = a.lo[1] + min(q, nrow - 1) % H;
+  int idx[3] = {i, j, k};
+  const R *T = rtab<2, R>(a.geo, k, idx);
+  const long c = (long)i * g.st[0] + (long)j * g.st[1] + k;
+  const bool full = i >= a.flo[0] && i <= a.fhi[0];
+  // one field at a time (a loop, not unrolled, so the loads of later fields
+  // are not hoisted into registers); left/right states wait in LDS
+  __shared__ R sl[9][64], sr[9][64];
+#pragma unroll 1
+  for (int m = 0; m < 9; m++) {
+    const R *Pm = m < 6 ? X + m * fs : M + (m - 6) * fs;
+    const R *Qm = m < 6 ? Xp + m * fs : Mp + (m - 6) * fs;
+    R xv[8], qv[8];
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) {
+      xv[s2] = Pm[lo_off[s2]];
+      qv[s2] = Qm[lo_off[s2]];
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) {
+      sx[8 * s2 + tx / 8][tx % 8] = xv[s2];
+      sh[8 * s2 + tx / 8][tx % 8] = half_step(xv[s2], qv[s2], cc);
+    }
+    __syncthreads();
+    R xn[8], xh[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      xn[n] = sx[tx][n];
+      xh[n] = sh[tx][n];
+    }
+    R l, r;
+    pdm_lr(xh, xn, T, l, r);
+    sl[m][tx] = l;
+    sr[m][tx] = r;
+    __syncthreads();  // the next field is staged over this one
+  }
+  R L[6], Rs[6], bl[3], br[3];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    L[m] = sl[m][tx];
+    Rs[m] = sr[m][tx];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    bl[m] = sl[6 + m][tx];
+    br[m] = sr[6 + m][tx];
+  }
+  if (!valid) return;
+  R *F = (R *)a.F, *V = (R *)a.V;
+  V[c] = R(0.5) * (L[3] + Rs[3]);
+  V[fs + c] = R(0.5) * (L[1] + Rs[1]);
+  if (!full) return;
+  R Fo[9];
+  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], a.b0, g.CA, Fo);
+#pragma unroll
+  for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_kcol(SweepArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
+  k_kcol_body<R>(a, bx, by, bz);
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_kcol_batch(const SweepArgs *__restrict__ args,
+                                                  const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_kcol_body<R>(args[arg], bx, by, bz);
+}
+
 // Points of a marching launch: the (row, k) plane of the launch box is walked
 // row by row, k fastest, 256 points per block (bx), so a k extent that is not
 // a multiple of 64 (257 faces or edges, small grids) does not leave waves
@@ -968,7 +1078,14 @@ struct EmfWaves {
   static constexpr int hi = (sizeof(R) == 8 && D != 0) ? 2 : 8;
 };
 
-template <int D, typename R>
+// STAGED (E_i and E_j, whose direct reconstructions run along k; the plan
+// uses it for FP64 E_i): a block is
+// one wave on 64 consecutive edges of one row; at each march position the
+// row's 72 cells of every directly reconstructed field are loaded once and
+// staged in LDS (current and half-step values), and each lane reads its
+// 8-cell stencil from there instead of loading it.  Whole 64-edge tiles only;
+// the unstaged form covers the rest.
+template <int D, typename R, bool STAGED = false>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
   const int chunk = a.chunk;
@@ -983,9 +1100,15 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   constexpr int O = P == MA ? Q : P;  // density axis other than the march axis
   constexpr bool VW = A2 == MA;       // v and F1 windowed
   const Grid &g = a.g;
-  (void)by;
   int k, r;
-  if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
+  if (STAGED) {
+    static_assert(!STAGED || D != 2, "E_k reconstructs across i, not k");
+    k = a.lo[2] + bx * 64 + (int)threadIdx.x;
+    r = a.lo[RA] + by;
+  } else {
+    (void)by;
+    if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
+  }
   const int f0 = a.lo[MA] + bz * chunk;
   const int f1 = min(f0 + chunk, a.hi[MA] + 1);
   int idx[3];
@@ -1024,6 +1147,13 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   constexpr int DA = VW ? R2 : A2;
   TableWalk<DA, MA, R> tw;
   tw.init(a.geo, idx[DA], idx, f0);
+  // staged form: LDS rows of the direct fields (E_i: v_a, v_b, then F1
+  // current and half-step; E_j: F2 current and half-step)
+  constexpr int NS = STAGED ? (VW ? 2 : 4) : 1;
+  __shared__ R stg[NS][KSPAN];
+  const int tx = threadIdx.x;
+  const long ka = (long)min(k - 4, g.N[2] - 1) - k;   // lane's first staged cell, offset from k
+  const long kb = (long)min(k + 60, g.N[2] - 1) - k;  // its halo cell (lanes < 8)
   // densities at march position f0 - 1 (offsets -1 and 0 along O)
   R u0 = rh(c - sm - so), u1 = rh(c - sm);
   for (int f = f0; f < f1; f++) {
@@ -1035,7 +1165,64 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
     march_load<1, R, R>(wb, Pw, Qw, o, cc, 7);
     const R *Td = tw.at(f, f1);
     R va, vb, b1l, b1r, b2l, b2r, l, rr;
-    if (VW) {
+    if constexpr (STAGED) {
+      // stage the row: every global load first, then the LDS writes
+      const R *S0 = VW ? B2 : V0, *S1 = VW ? B2p : V1;
+      R ya[4], yb[4];
+      ya[0] = S0[c + ka];
+      ya[1] = S1[c + ka];
+      if (!VW) {
+        ya[2] = B1[c + ka];
+        ya[3] = B1p[c + ka];
+      }
+      if (tx < KSPAN - 64) {
+        yb[0] = S0[c + kb];
+        yb[1] = S1[c + kb];
+        if (!VW) {
+          yb[2] = B1[c + kb];
+          yb[3] = B1p[c + kb];
+        }
+      }
+      auto put = [&](const R v[4], int at) {
+        if (VW) {  // F2: current, half-step
+          stg[0][at] = v[0];
+          stg[1][at] = half_step(v[0], v[1], cc);
+        } else {   // v_a, v_b, F1 current, F1 half-step
+          stg[0][at] = v[0];
+          stg[1][at] = v[1];
+          stg[2][at] = v[2];
+          stg[3][at] = half_step(v[2], v[3], cc);
+        }
+      };
+      put(ya, tx);
+      if (tx < KSPAN - 64) put(yb, 64 + tx);
+      __syncthreads();
+      auto lds_lr = [&](int sx, int sh, R &L, R &Rr) {
+        R xh[8], xn[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          xn[n] = stg[sx][tx + n];
+          xh[n] = stg[sh][tx + n];
+        }
+        pdm_lr(xh, xn, Td, L, Rr);
+      };
+      if (VW) {
+        lr_self(wv0, Tm, l, rr);
+        va = R(0.5) * (l + rr);
+        lr_self(wv1, Tm, l, rr);
+        vb = R(0.5) * (l + rr);
+        march_lr(wb, 0, Tm, b1l, b1r);
+        lds_lr(0, 1, b2l, b2r);
+      } else {
+        lds_lr(0, 0, l, rr);
+        va = R(0.5) * (l + rr);
+        lds_lr(1, 1, l, rr);
+        vb = R(0.5) * (l + rr);
+        lds_lr(2, 3, b1l, b1r);
+        march_lr(wb, 0, Tm, b2l, b2r);
+      }
+      __syncthreads();  // the next position is staged over this one
+    } else if (VW) {
       lr_self(wv0, Tm, l, rr);
       va = R(0.5) * (l + rr);
       lr_self(wv1, Tm, l, rr);
@@ -1077,6 +1264,27 @@ __global__ __launch_bounds__(256)
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
   k_emf_march_body<D, R>(a, bx, by, bz);
+}
+
+// staged form over whole 64-edge tiles along k (E_i, E_j): one wave per block
+template <int D, typename R>
+__global__ __launch_bounds__(64) void k_emf_stage(EmfArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
+  k_emf_march_body<D, R, true>(a, bx, by, bz);
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restrict__ args,
+                                                       const WorkItem *__restrict__ work,
+                                                       int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
+  const int by = __builtin_amdgcn_readfirstlane(work[w].by);
+  const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
+  k_emf_march_body<D, R, true>(args[arg], bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
@@ -2185,11 +2393,11 @@ void resolve_timers(gk_solver *s) {
 // merged per stage into one work-list launch).
 enum StageKernel {
   SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
-  SK_EMF_K, SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
+  SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
 };
 static const char *const sk_name[SK_COUNT] = {
     "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
-    "k_emf_j", "k_emf_k", "k_ct", "k_cell", "k_ghost", "k_mags"};
+    "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_ct", "k_cell", "k_ghost", "k_mags"};
 
 union StageArgs {
   SweepArgs sw;
@@ -2206,7 +2414,8 @@ static size_t stage_arg_size(int kid) {
     case SK_BEGIN: return sizeof(Scal *);
     case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
       return sizeof(SweepArgs);
-    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return sizeof(EmfArgs);
+    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: case SK_EMF_IS: case SK_EMF_JS:
+      return sizeof(EmfArgs);
     case SK_CT: return sizeof(CtArgs);
     case SK_CELL: return sizeof(CellArgs);
     case SK_GHOST: return sizeof(GhostArgs);
@@ -2219,7 +2428,8 @@ static const void *stage_arg_ptr(int kid, const StageArgs &u) {
     case SK_BEGIN: return &u.sc;
     case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
       return &u.sw;
-    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: return &u.em;
+    case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: case SK_EMF_IS: case SK_EMF_JS:
+      return &u.em;
     case SK_CT: return &u.ct;
     case SK_CELL: return &u.ce;
     case SK_GHOST: return &u.gh;
@@ -2340,8 +2550,9 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       if (split) {
         SweepArgs v = a;
         v.lo[2] = m.hi[2] + 1;
-        L.kid = SK_KREM;
-        L.grid = dim3(point_blocks(v), 1, 1);
+        L.kid = SK_KREM;  // k_kcol: one block per column and 64 rows
+        const long nrow = (long)(v.hi[1] - v.lo[1] + 1) * (v.hi[0] - v.lo[0] + 1);
+        L.grid = dim3((unsigned)((nrow + 63) / 64 * rem), 1, 1);
         L.u.sw = v;
         out.push_back(L);
       }
@@ -2369,11 +2580,30 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     }
     const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
     const int ce = s->emf_chunk;
-    L.kid = SK_EMF_I + D;
-    L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA]), 1,
-                  (a.hi[MA] - a.lo[MA] + ce) / ce);
-    L.u.em = a;
-    out.push_back(L);
+    const int nf = (a.hi[MA] - a.lo[MA] + ce) / ce;
+    // FP64 E_i: whole 64-edge tiles along k in the staged form, the rest (all
+    // of it when there is no whole tile) in the marching form.  Staging
+    // measured slower for E_j (one direct field) and for the FP32 form.
+    const int ntile = (D == 0 && !f32) ? (a.hi[2] - a.lo[2] + 1) / 64 : 0;
+    if (ntile > 0) {
+      EmfArgs t = a;
+      t.hi[2] = a.lo[2] + 64 * ntile - 1;
+      L.kid = D == 0 ? SK_EMF_IS : SK_EMF_JS;
+      L.grid = dim3(ntile, a.hi[RA] - a.lo[RA] + 1, nf);
+      L.u.em = t;
+      out.push_back(L);
+      a.lo[2] = t.hi[2] + 1;
+      // the leftover columns are few points: march one edge per thread so
+      // that they spread over many waves
+      a.chunk = 1;
+    }
+    if (a.lo[2] <= a.hi[2]) {
+      L.kid = SK_EMF_I + D;
+      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA]), 1,
+                    (a.hi[MA] - a.lo[MA] + a.chunk) / a.chunk);
+      L.u.em = a;
+      out.push_back(L);
+    }
   }
   {
     CtArgs a;
@@ -2457,10 +2687,12 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_KREM: k_sweep<2, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_KREM: k_kcol<R><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_EMF_I: k_emf_march<0, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_EMF_J: k_emf_march<1, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_EMF_K: k_emf_march<2, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_IS: k_emf_stage<0, R><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_JS: k_emf_stage<1, R><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_CT: k_ct<R><<<g1, blk, 0, st>>>(L.u.ct, gx, gy, gz); break;
     case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
@@ -2478,10 +2710,12 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
     case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_KREM: k_sweep_batch<2, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_KREM: k_kcol_batch<R><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_EMF_I: k_emf_march_batch<0, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_EMF_J: k_emf_march_batch<1, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_EMF_K: k_emf_march_batch<2, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_IS: k_emf_stage_batch<0, R><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_JS: k_emf_stage_batch<1, R><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_CT: k_ct_batch<R><<<nblocks, blk, 0, st>>>((const CtArgs *)args, work, nwork); break;
     case SK_CELL: k_cell_batch<R><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
     case SK_GHOST: k_ghost_batch<<<nblocks, blk, 0, st>>>((const GhostArgs *)args, work, nwork); break;
