@@ -1397,7 +1397,7 @@ struct CellArgs {
   const double *X, *Xp;    // X^n, X^{n-1}
   double *Xn;              // X^{n+1} (aliases Xp: each cell reads before it writes)
   float *Xn32;             // FP32 copy of X^{n+1} (FP32-bulk form), else null
-  const double *M, *Mp;    // cell B at n, n-1
+  const void *M, *Mp;      // cell B at n, n-1, of type R (FP64, or the FP32 copies)
   const double *Bn;        // face B at n+1
   const void *F[3];        // 9 fields of R per direction
   Scal *sc;
@@ -1465,7 +1465,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     for (int m = 0; m < 6; m++) x[m] = a.X[m * g.fs + c];
     double bold[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) bold[m] = a.M[m * g.fs + c];
+    for (int m = 0; m < 3; m++) bold[m] = (double)((const R *)a.M)[m * g.fs + c];
     // face B at n+1 on the cell's six faces (cell-centred B and div B below),
     // loaded with the other inputs so the wave waits for memory once
     double bf[3][2];
@@ -1505,7 +1505,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
       double bh[3];
 #pragma unroll
       for (int m = 0; m < 3; m++) {
-        double bp = a.Mp[m * g.fs + c];
+        double bp = (double)((const R *)a.Mp)[m * g.fs + c];
         bh[m] = bold[m] + cc * (bold[m] - bp);
       }
       const double *c0 = a.b0.cell;
@@ -1811,7 +1811,7 @@ struct MagsArgs {
   Grid g;
   Geo geo;
   const double *B;
-  double *M;
+  double *M;   // or null (not written)
   float *M32;  // FP32 copy (FP32-bulk form), else null
 };
 
@@ -1831,7 +1831,7 @@ __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, con
   for (int m = 0; m < 3; m++) b[m] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
-    a.M[m * g.fs + c] = b[m];
+    if (a.M) a.M[m * g.fs + c] = b[m];
     if (a.M32) a.M32[m * g.fs + c] = (float)b[m];
   }
 }
@@ -2253,6 +2253,9 @@ static void fill_x64_ghosts(gk_solver *s) {
 static void refresh_copies32(gk_solver *s) {
   if (s->precision != 1) return;
   fill_x64_ghosts(s);
+  // FP32-bulk steps keep only the FP32 copy of the cell-centred B: rebuild
+  // the FP64 one from the face field (cell B is a function of face B alone)
+  for (int b = 0; b < 3; b++) launch_mags(s, s->B[b], s->M[b]);
   const size_t fs = s->fs;
   const dim3 grid(2048), blk(256);
   for (int b = 0; b < 2; b++) hipLaunchKernelGGL(k_to32, grid, blk, 0, s->stream, s->X[b], s->X32[b], 6 * fs);
@@ -2631,8 +2634,10 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.Xp = Xp;
     a.Xn = Xn;
     a.Xn32 = f32 ? s->X32[1 - r.cur] : nullptr;
-    a.M = M;
-    a.Mp = Mm1;
+    // the cell-centred B of the Boris correction and the spherical sources:
+    // the FP32-bulk form reads the FP32 copies and keeps no FP64 cell B
+    a.M = Ms;
+    a.Mp = Mps;
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
@@ -2664,7 +2669,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.B = Bn;
-    a.M = Mn;
+    a.M = f32 ? nullptr : Mn;  // FP32-bulk: only the FP32 copy is read
     a.M32 = f32 ? s->M32[r.b_n] : nullptr;
     L.kid = SK_MAGS;
     L.grid = mags_grid(g);
@@ -2793,6 +2798,8 @@ extern "C" int gk_debug_fetch(gk_solver *s, int which, int field, double *host) 
   if (which >= 3 && which < 6 && field < 2) src = (const char *)s->V[which - 3] + field * fb;
   if (which == 6 && field < 3) src = (const char *)s->E + field * fb;
   if (which == 7 && field < 3) {
+    // FP32-bulk steps keep only the FP32 copy: form the FP64 one first
+    if (s->precision == 1) launch_mags(s, s->B[s->b_0], s->M[s->b_0]);
     src = (const char *)(s->M[s->b_0] + field * s->fs);
     esz = sizeof(double);
   }
