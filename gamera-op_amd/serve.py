@@ -140,6 +140,10 @@ def _worker_init(paths):
     for p in paths:
         if p not in sys.path:
             sys.path.insert(0, p)
+    # load the host libraries and run one tiny set-up now, so that the first
+    # request a worker prepares does not pay for it
+    import gamera_op as gw
+    gw.Problem(gw.SINUSOID, 1, 8, 8, 8).close()
 
 
 def _worker_prepare(kind, seed, shape):
