@@ -1398,6 +1398,8 @@ struct CellArgs {
   double *Xn;              // X^{n+1} (aliases Xp: each cell reads before it writes)
   float *Xn32;             // FP32 copy of X^{n+1} (FP32-bulk form), else null
   const void *M, *Mp;      // cell B at n, n-1, of type R (FP64, or the FP32 copies)
+  double *Mn;              // cell B at n+1 of the active cells (FP64 form), else null
+  float *Mn32;             // its FP32 copy (FP32-bulk form), else null
   const double *Bn;        // face B at n+1
   const void *F[3];        // 9 fields of R per direction
   Scal *sc;
@@ -1574,6 +1576,12 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
 #pragma unroll
       for (int m = 0; m < 6; m++) a.Xn32[m * g.fs + c] = (float)out[m];
     }
+    // cell-centred B at n+1 (the ghost cells' is formed after the ghost fill)
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      if (a.Mn) a.Mn[m * g.fs + c] = bnew[m];
+      if (a.Mn32) a.Mn32[m * g.fs + c] = (float)bnew[m];
+    }
     // next step's CFL dt from the new state (magtot = cell B + B0)
     double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0],
                           bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2], dxi, dxj, dxk, CA);
@@ -1689,9 +1697,12 @@ __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o
 // The points of the box outside the active cells, walked as three slabs:
 // i outside [s0, e0] (all j, k), then j outside [s1, e1] (i active, all k),
 // then k outside [s2, e2] (i, j active); k fastest in each, 256 per block.
-__device__ __forceinline__ bool ghost_point(const Grid &g, int bx, int &i, int &j, int &k) {
+// The box is N - shrink points per axis (shrink 1 leaves out the upper face
+// slot, for cell quantities formed from both faces of a cell).
+__device__ __forceinline__ bool ghost_point(const Grid &g, int bx, int &i, int &j, int &k,
+                                           int shrink = 0) {
   const int n0 = g.e[0] - g.s[0] + 1, n1 = g.e[1] - g.s[1] + 1, n2 = g.e[2] - g.s[2] + 1;
-  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const int N0 = g.N[0] - shrink, N1 = g.N[1] - shrink, N2 = g.N[2] - shrink;
   long p = (long)bx * 256 + threadIdx.y * 64 + threadIdx.x;
   const long ca = (long)(N0 - n0) * N1 * N2, cb = (long)n0 * (N1 - n1) * N2;
   const long cc = (long)n0 * n1 * (N2 - n2);
@@ -1815,13 +1826,19 @@ struct MagsArgs {
   float *M32;  // FP32 copy (FP32-bulk form), else null
 };
 
+template <bool SHELL = false>
 __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
                                        const int bz) {
   const Grid &g = a.g;
   (void)by;
-  int k, j;
-  if (!row_point(0, g.N[2] - 2, 0, g.N[1] - 2, bx, k, j)) return;
-  const int i = bz;
+  int i, j, k;
+  if (SHELL) {  // only the points outside the active cells (the cell update forms the rest)
+    (void)bz;
+    if (!ghost_point(g, bx, i, j, k, 1)) return;
+  } else {
+    if (!row_point(0, g.N[2] - 2, 0, g.N[1] - 2, bx, k, j)) return;
+    i = bz;
+  }
   const long c = i * g.st[0] + j * g.st[1] + k;
   const int idx[3] = {i, j, k};
   // all three components before the first store (a load after a store
@@ -1865,6 +1882,14 @@ __global__ __launch_bounds__(256) void k_mags(MagsArgs a, int gx, int gy, int gz
   k_mags_body(a, bx, by, bz);
 }
 
+// cell-centred B of the points outside the active cells (a step's cell
+// update writes the active ones)
+__global__ __launch_bounds__(256) void k_mags_shell(MagsArgs a, int gx, int gy, int gz) {
+  int bx, by, bz;
+  if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
+  k_mags_body<true>(a, bx, by, bz);
+}
+
 // one launch for many grids: block w runs work-list entry w (argument set, block)
 __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
@@ -1875,7 +1900,7 @@ __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_mags_body(args[arg], bx, by, bz);
+  k_mags_body<true>(args[arg], bx, by, bz);
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -2209,6 +2234,17 @@ static dim3 ghost_grid(const Grid &g) {
 // blocks a launch decoded by xcd_patch needs (whole 8 x 8 patches)
 static int patch_blocks(const dim3 &g) {
   return (int)((g.x + PATCH - 1) / PATCH * PATCH * ((g.z + PATCH - 1) / PATCH) * PATCH);
+}
+
+// cell-centred B of the points outside the active cells, in the box minus
+// the upper face slot
+static dim3 mags_shell_grid(const Grid &g) {
+  long n = 1, t = 1;
+  for (int d = 0; d < 3; d++) {
+    n *= g.e[d] - g.s[d] + 1;
+    t *= g.N[d] - 1;
+  }
+  return dim3((unsigned)((t - n + 255) / 256), 1, 1);
 }
 
 // cell-centred B over the box minus the upper face slot
@@ -2638,6 +2674,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     // the FP32-bulk form reads the FP32 copies and keeps no FP64 cell B
     a.M = Ms;
     a.Mp = Mps;
+    a.Mn = f32 ? nullptr : Mn;
+    a.Mn32 = f32 ? s->M32[r.b_n] : nullptr;
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
@@ -2671,8 +2709,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.B = Bn;
     a.M = f32 ? nullptr : Mn;  // FP32-bulk: only the FP32 copy is read
     a.M32 = f32 ? s->M32[r.b_n] : nullptr;
-    L.kid = SK_MAGS;
-    L.grid = mags_grid(g);
+    L.kid = SK_MAGS;  // ghost cells only (k_mags_shell)
+    L.grid = mags_shell_grid(g);
     L.u.mg = a;
     out.push_back(L);
   }
@@ -2683,7 +2721,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 blk(64, 4);
   const int gx = L.grid.x, gy = L.grid.y, gz = L.grid.z;
   // decoded by xcd_patch (cell update, CT, cell B) or xcd_block (the others)
-  const bool patch = L.kid == SK_CT || L.kid == SK_CELL || L.kid == SK_MAGS;
+  const bool patch = L.kid == SK_CT || L.kid == SK_CELL;
   const int total = patch ? patch_blocks(L.grid) : gx * gy * gz;
   const dim3 g1((total + NXCD - 1) / NXCD * NXCD);
   switch (L.kid) {
@@ -2701,7 +2739,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_CT: k_ct<R><<<g1, blk, 0, st>>>(L.u.ct, gx, gy, gz); break;
     case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
-    case SK_MAGS: k_mags<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
+    case SK_MAGS: k_mags_shell<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
   }
 }
 
