@@ -17,6 +17,10 @@ MIX = [
     (go.DIPOLE_SHELL, 13, 16, 12, 16, 0),
     (go.SINUSOID_SPHERE, 14, 16, 8, 8, 1),
     (go.SINUSOID_BLAST, 15, 32, 24, 16, 0),
+    # k extents past one 64-wide tile: staged E_i tiles and the leftover
+    # k columns (k faces, E_i edges) in both forms
+    (go.SINUSOID_BLAST, 16, 12, 8, 66, 0),
+    (go.SINUSOID, 17, 8, 12, 65, 1),
 ]
 
 

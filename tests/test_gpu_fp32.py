@@ -25,6 +25,8 @@ CASES = [
     (go.SINUSOID_BLAST, 3, 32, 32, 32, 50),
     (go.DIPOLE_SHELL, 4, 24, 16, 24, 40),
     (go.SINUSOID_SPHERE, 5, 24, 16, 8, 40),
+    # 65 k faces: a 64-face tile of the staged k sweep plus a leftover column
+    (go.SINUSOID_BLAST, 6, 16, 12, 64, 30),
 ]
 
 
