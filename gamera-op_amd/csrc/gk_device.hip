@@ -440,11 +440,10 @@ struct SweepArgs {
   int chunk;             // faces per thread in the marching sweeps
 };
 
-// Direct sweep for narrow k ranges: the velocity-only strips of the j sweep
-// (k in the four ghost layers on either side) and the k faces left over when
-// the k extent is not a multiple of the 64-face tiles of k_sweepk.  Each
-// thread reads its eight stencil cells; the points of the launch box are
-// walked k fastest, then j, then i, 256 per block.
+// Direct sweep for the velocity-only strips of the j sweep (k in the four
+// ghost layers on either side).  Each thread reads its eight stencil cells;
+// the points of the launch box are walked k fastest, then j, then i, 256 per
+// block.
 template <int D, typename R>
 __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
