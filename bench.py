@@ -60,12 +60,14 @@ def kernel_alg_bytes(name, n):
         # 3 face-B in, 3 EMF in, 3 face-B out
         return 8 * 9 * full_faces
     if name == "k_cell":
-        # 27 face fluxes, X^n (6), cell B^n (3), face B^{n+1} (3) in; X^{n+1} out
-        return 8 * n ** 3 * 45
+        # 27 face fluxes, X^n (6), cell B^n (3), face B^{n+1} (3) in; X^{n+1} (6)
+        # and cell B^{n+1} (3) out
+        return 8 * n ** 3 * 48
     if name == "k_ghost":
         return 8 * 2 * 9 * (N ** 3 - n ** 3)
     if name == "k_mags":
-        return 8 * 6 * (N - 1) ** 3
+        # cell B of the points outside the active cells: 3 face fields in, 3 out
+        return 8 * 6 * ((N - 1) ** 3 - n ** 3)
     return 0
 
 
