@@ -2131,7 +2131,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     return fail(GK_ERR_DEVICE);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
   s->chunk = 32;
-  s->emf_chunk = 8;
+  s->emf_chunk = 16;
   s->cur = 0;
   s->b_m1 = 0;
   s->b_0 = 1;

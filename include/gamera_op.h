@@ -127,7 +127,7 @@ size_t gk_device_bytes(const gk_solver *s);
 /* Launch-shape parameters (no counterpart in the reference; they change
    speed, never results).  GK_TUNE_CHUNK: faces each thread walks in the
    i and j sweeps (1..4096, default 32); GK_TUNE_EMF_CHUNK: edges each thread
-   walks in the edge-field kernels (1..4096, default 8). */
+   walks in the edge-field kernels (1..4096, default 16). */
 enum { GK_TUNE_CHUNK = 0, GK_TUNE_EMF_CHUNK = 1 };
 int gk_set_tuning(gk_solver *s, int param, int value);
 int gk_get_tuning(const gk_solver *s, int param);
