@@ -331,7 +331,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     return results, wall
 
 
-def serve_plain(requests, tune=False, digests=False, realtime=True):
+def serve_plain(requests, tune=False, digests=False, realtime=True, log=None):
     """Batching off: one request at a time, in arrival order."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     setup = _Setup()
@@ -353,6 +353,8 @@ def serve_plain(requests, tune=False, digests=False, realtime=True):
                 res.digest = state_digest(S)
             results.append(res)
             S.close()
+            if log:
+                log(f"done rid={r.rid} latency={res.latency:.3f}s")
     finally:
         setup.close()
     return results, time.perf_counter() - t0
@@ -366,3 +368,57 @@ def summarize(results, wall):
             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
             "latency_max_s": float(lat.max()) if len(lat) else None}
+
+
+def main(argv=None):
+    """Command line: serve a seeded request stream (the configured mix) on one
+    GPU and print one JSON line with throughput, latency percentiles and the
+    loop's time breakdown; --plain also runs the batching-off loop on the same
+    requests."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=5.0, help="mean arrivals per second")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--budget-gib", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=0, help="preparation processes (0: auto)")
+    ap.add_argument("--plain", action="store_true", help="also run the plain loop")
+    ap.add_argument("--no-serve", action="store_true", help="run only the plain loop")
+    ap.add_argument("--progress", type=int, default=50,
+                    help="print a line to stderr every N finished requests")
+    args = ap.parse_args(argv)
+
+    reqs = make_requests(args.requests, args.rate, seed=args.seed)
+    out = {"requests": len(reqs), "arrival_rate_per_s": args.rate, "seed": args.seed,
+           "budget_GiB": args.budget_gib}
+    done = [0]
+
+    def progress(msg):
+        done[0] += 1
+        if args.progress and done[0] % args.progress == 0:
+            print(f"serve: {done[0]} of {len(reqs)} finished ({msg})", file=sys.stderr, flush=True)
+
+    if not args.no_serve:
+        preparer = Preparer(args.workers or None)  # before this process touches the GPU
+        try:
+            res, wall = serve(reqs, budget_bytes=args.budget_gib << 30, preparer=preparer,
+                              log=progress)
+        finally:
+            preparer.close()
+        out["served"] = summarize(res, wall)
+        out["served"]["loop_seconds"] = {k: round(v, 3) if isinstance(v, float) else v
+                                         for k, v in serve.last_stats.items()}
+    if args.plain or args.no_serve:
+        done[0] = 0
+        res, wall = serve_plain(reqs, log=progress)
+        out["plain_loop"] = summarize(res, wall)
+    if "served" in out and "plain_loop" in out:
+        out["throughput_ratio"] = (out["served"]["cell_updates_per_s"] /
+                                   out["plain_loop"]["cell_updates_per_s"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
