@@ -503,3 +503,36 @@ GK_BC_DEF(gem, 1, reflective, GK_BCK_NONE)
 GK_BC_DEF(gem, 1, fixed, GK_BCK_FIXED)
 GK_BC_DEF(gem, 1, extrapolated, GK_BCK_EXTRAPOLATED)
 GK_BC_DEF(gem, 1, none, GK_BCK_NONE)
+
+/* Pole boundary on a j side (src/spherical/spherical_boundary.c:23-138):
+   ghost rows from the row mirrored across the pole, rotated by pi in phi, for
+   every i and k of the box; v2, v3 and B_j, B_k, b2, b3 change sign. */
+static void pole_mirror(int is_gem, int side) {
+  const int NJ = config.NJ, nk = config.nk_global;
+  for (int f = 0; f < 6; f++) {
+    double ***a = is_gem ? gem[f] : gas[0][f];
+    int o = is_gem ? gem_onface_j[f] : gas_onface_j[f];
+    int neg = is_gem ? !(f == mag_bi || f == mags_b1) : (f == gas_v2 || f == gas_v3);
+    for (int i = 0; i < config.NI; i++)
+      for (int g = 0; g < NG; g++) {
+        int dst = side == 0 ? g : NJ - 1 - NG + g + o;
+        int src = side == 0 ? 2 * NG - 1 - g + o : NJ - 1 - NG - 1 - g;
+        for (int k = 0; k < config.NK; k++) {
+          double v = a[i][src][NG + (k - NG + nk / 2) % nk];
+          a[i][dst][k] = neg ? -v : v;
+        }
+      }
+  }
+}
+
+static void bc_pole_request(int is_gem, int side) {
+  if (dim_periodic(1)) return;
+  gk_bc_kind[is_gem][2 + side] = GK_BCK_POLE;
+  if (gk_bc_record_only || !gas || !gem) return;
+  pole_mirror(is_gem, side);
+}
+
+void gas_bc_pole_j_low() { bc_pole_request(0, 0); }
+void gas_bc_pole_j_high() { bc_pole_request(0, 1); }
+void gem_bc_pole_j_low() { bc_pole_request(1, 0); }
+void gem_bc_pole_j_high() { bc_pole_request(1, 1); }

@@ -18,7 +18,13 @@ enum {
   GK_BCK_FIXED,
   GK_BCK_EXTRAPOLATED,
   GK_BCK_NONE,
+  GK_BCK_POLE,
 };
+
+/* ring-average blocks per row for nk azimuthal cells
+   (src/spherical/spherical_RingAverage.c:55-79): returns the number of
+   averaged rows (NAVR) and fills nchunk, or -1 for an unsupported nk */
+int gk_ring_table(int nk, int nchunk[16]);
 extern int gk_bc_kind[2][6];
 extern int gk_bc_record_only;
 

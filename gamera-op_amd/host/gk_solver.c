@@ -217,7 +217,10 @@ double get_dt() {
         double Vf = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
         double cs = sqrt(gamma_val * gas[0][gas_p][i][j][k] / rho);
         double V = Vf + sqrt(cs * cs + Vae * Vae);
-        double d = CFL / (V / geo[dxi][i][j][k] + V / geo[dxj][i][j][k] + V / geo[dxk][i][j][k]);
+        /* get_dt_ring (src/spherical/spherical_general.c:700-735) uses the
+           ring-widened azimuthal width */
+        const int qk = doRingAverage == 1 ? dxk_ring : dxk;
+        double d = CFL / (V / geo[dxi][i][j][k] + V / geo[dxj][i][j][k] + V / geo[qk][i][j][k]);
         m = fmin(m, d);
       }
   return m;
@@ -293,6 +296,7 @@ gk_solver *gkh_device(void) { return g_dev; }
 typedef struct {
   uint32_t magic, version;
   int ni, nj, nk, coord, bc[2][6], b0_mode, N[3], ncls[3][2];
+  int ring, ring_pole[2], ring_rows, ring_nchunk[16];
   double CA, b0_cell[3], b0_face[3][8], b0_edge[3][3], b0_lorentz[3];
 } gk_blob_head;
 #define GK_BLOB_MAGIC 0x6b426c6fu
@@ -335,7 +339,11 @@ long gkh_desc_blob(void *buf, size_t cap) {
   gk_blob_head h;
   memset(&h, 0, sizeof h);
   h.magic = GK_BLOB_MAGIC;
-  h.version = 1;
+  h.version = 2;
+  h.ring = d.ring;
+  memcpy(h.ring_pole, d.ring_pole, sizeof h.ring_pole);
+  h.ring_rows = d.ring_rows;
+  memcpy(h.ring_nchunk, d.ring_nchunk, sizeof h.ring_nchunk);
   h.ni = d.ni;
   h.nj = d.nj;
   h.nk = d.nk;
@@ -396,7 +404,7 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   *out = NULL;
   gk_blob_head h;
   memcpy(&h, blob, sizeof h);
-  if (h.magic != GK_BLOB_MAGIC || h.version != 1) return GK_ERR_ARG;
+  if (h.magic != GK_BLOB_MAGIC || h.version != 2) return GK_ERR_ARG;
   size_t off[GK_G_COUNT + 5];
   if (blob_layout(&h, off) > n) return GK_ERR_ARG;
   const char *b = (const char *)blob;
@@ -439,6 +447,10 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   memcpy(d.b0_face, h.b0_face, sizeof d.b0_face);
   memcpy(d.b0_edge, h.b0_edge, sizeof d.b0_edge);
   memcpy(d.b0_lorentz, h.b0_lorentz, sizeof d.b0_lorentz);
+  d.ring = h.ring;
+  memcpy(d.ring_pole, h.ring_pole, sizeof d.ring_pole);
+  d.ring_rows = h.ring_rows;
+  memcpy(d.ring_nchunk, h.ring_nchunk, sizeof d.ring_nchunk);
   gk_solver *s = NULL;
   int rc = gk_create(&d, &s);
   if (rc != GK_OK) return rc;
@@ -647,8 +659,8 @@ int gkh_main(const char *config_file, int nt) {
   A1 = A2 = A3 = Field_Default;
   set_problem_config();
   /* coordinate system of the seeded problem: spherical for the dipole
-     shell (kind 2), Cartesian for the others */
-  gk_coord_type = gk_problem_kind == 2 ? 2 : 0;
+     shell (kind 2) and the full sphere (kind 4), Cartesian for the others */
+  gk_coord_type = (gk_problem_kind == 2 || gk_problem_kind == 4) ? 2 : 0;
   if (initialize_config(0, 1, config_file) != 0) return -1;
   if (setup_comm_and_types(0) != 0) return -1;
   if (initialize_logging(BASE_LOG_NAME, 0) != 0) return -1;
@@ -656,6 +668,10 @@ int gkh_main(const char *config_file, int nt) {
   getweights();
   if (!read_restart) problem_init();
   set_background_field();
+  if (doRingAverage == 1) { /* src/main.c:66-70 */
+    set_RingAverage_geo_data();
+    set_RingAverage_config();
+  }
   g_dev_fresh = 1;
   if (nt > 0 && solve(nt) != 0) return -3;
   return 0;
@@ -693,6 +709,10 @@ int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int fu
   gk_b0_full_arrays = full_b0;
   set_background_field();
   if (gk_b0.mode < 0) return -3;
+  if (doRingAverage == 1) {
+    set_RingAverage_geo_data();
+    set_RingAverage_config();
+  }
   record_bcs();
   g_dev_fresh = 1;
   return 0;

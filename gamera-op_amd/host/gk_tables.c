@@ -22,6 +22,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "log.h"
+
 #include "config.h"
 #include "curvilinear.h"
 #include "gk_host.h"
@@ -205,6 +207,7 @@ static int dev_bc(int dim, int side, int arr) {
   if (b == BC_PERIODIC) return GK_BC_PERIODIC;
   switch (gk_bc_kind[arr][2 * dim + side]) {
     case GK_BCK_SYMMETRIC: return GK_BC_SYMMETRIC;
+    case GK_BCK_POLE: return dim == 1 ? GK_BC_POLE : -1;
     case GK_BCK_UNSET:
     case GK_BCK_NONE: return GK_BC_NONE;
     default: return -1;
@@ -241,5 +244,49 @@ int gk_fill_desc(gk_grid_desc *d, const gk_tables *t, int precision) {
   memcpy(d->b0_cell, gk_b0.cell, sizeof d->b0_cell);
   memcpy(d->b0_face, gk_b0.face, sizeof d->b0_face);
   memcpy(d->b0_edge, gk_b0.edge, sizeof d->b0_edge);
+  if (doRingAverage == 1) {
+    d->ring = 1;
+    d->ring_rows = gk_ring_table(config.nk_global, d->ring_nchunk);
+    if (d->ring_rows < 0) return GK_ERR_ARG;
+    d->ring_pole[0] = fabs(x2min_global) < 1e-10;
+    d->ring_pole[1] = fabs(x2max_global - PI) < 1e-10;
+  }
   return GK_OK;
+}
+
+int gk_ring_table(int nk, int nchunk[16]) {
+  static const int t[16] = {8, 16, 32, 32, 64, 64, 64, 64, 128, 128, 128, 128, 128, 128, 128, 128};
+  int n = nk == 256 ? 16 : nk == 128 ? 8 : nk == 64 ? 4 : nk == 32 ? 2 : -1;
+  for (int r = 0; r < 16; r++) nchunk[r] = r < n ? t[r] : 0;
+  return n;
+}
+
+/* Ring-average set-up of the reference main() (src/main.c:66-70).  The face
+   areas of the ring rows are the ordinary ones (the reference copies them,
+   src/spherical/spherical_RingAverage.c:32-51), so only the CFL width
+   geo[dxk_ring] and the pole flags are set here (:54-147). */
+void set_RingAverage_geo_data() {}
+
+void set_RingAverage_config() {
+  int nch[16];
+  NAVR = gk_ring_table(config.nk_global, nch);
+  if (NAVR < 0) {
+    log_error("config.nk_global must be 32, 64, 128, or 256");
+    return;
+  }
+  for (int r = 0; r < NAVR; r++) Nchunk[r] = nch[r];
+  for (int i = isg; i <= ieg; i++)
+    for (int j = jsg; j <= jeg; j++)
+      for (int k = ksg; k <= keg; k++) geo[dxk_ring][i][j][k] = geo[dxk][i][j][k];
+  doRingAverage_thisrank_positivepole = fabs(x2min_global) <= 1e-10;
+  doRingAverage_thisrank_negativepole = fabs(x2max_global - PI) <= 1e-10;
+  for (int p = 0; p < 2; p++) {
+    if (!(p == 0 ? doRingAverage_thisrank_positivepole : doRingAverage_thisrank_negativepole)) continue;
+    for (int r = 0; r < NAVR && r < config.nj; r++) {
+      int Nm = config.nk_global / Nchunk[r];
+      int j = p == 0 ? js + r : je - r;
+      for (int i = is; i <= ie; i++)
+        for (int k = ks; k <= ke; k++) geo[dxk_ring][i][j][k] = Nm * geo[dxk][i][j][k];
+    }
+  }
 }

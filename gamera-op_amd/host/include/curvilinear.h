@@ -27,4 +27,13 @@ void curvilinear_to_cartesian_vector(double x1, double x2, double x3,
                                      double v1, double v2, double v3,
                                      double *vx, double *vy, double *vz);
 
+/* pole boundary and ring-average set-up (spherical grids reaching a pole;
+   src/curvilinear.h:80-99) */
+void gas_bc_pole_j_low();
+void gas_bc_pole_j_high();
+void gem_bc_pole_j_low();
+void gem_bc_pole_j_high();
+void set_RingAverage_geo_data();
+void set_RingAverage_config();
+
 #endif

@@ -23,6 +23,15 @@
  *                      potential A_phi = m sin(theta)/r^2; rho=1, p=0.1, plus an
  *                      over-pressure blob (R=0.5, p ratio 50).
  *   GK_SINUSOID_SPHERE GK_SINUSOID + the over-pressure sphere, no B0.
+ *   GK_POLE_SPHERE     Full sphere: log-spaced r in [1,10], theta in [0, pi]
+ *                      (both poles), phi periodic, pole ring averaging on
+ *                      (doRingAverage = 1, as src/problem_FieldLoop_spherical.c
+ *                      sets it) with the pole boundary on both theta sides;
+ *                      dipole B as GK_DIPOLE_SHELL, rho = 1 + 0.1 xi, a uniform
+ *                      Cartesian flow of speed 0.3 in a seeded direction (it
+ *                      crosses the poles, so every pole ring carries azimuthal
+ *                      structure), p = 0.1 plus an over-pressure blob (ratio 50)
+ *                      centred near the north pole.
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -46,7 +55,16 @@ enum {
   GK_SINUSOID_BLAST = 1,
   GK_DIPOLE_SHELL = 2,
   GK_SINUSOID_SPHERE = 3,
+  GK_POLE_SPHERE = 4,
 };
+
+/* Pole boundary routines exist only in spherical builds of the reference
+   (src/spherical/spherical_boundary.c); weak references keep this plugin
+   linkable into a Cartesian build, where GK_POLE_SPHERE is not available. */
+void gas_bc_pole_j_low(void) __attribute__((weak));
+void gas_bc_pole_j_high(void) __attribute__((weak));
+void gem_bc_pole_j_low(void) __attribute__((weak));
+void gem_bc_pole_j_high(void) __attribute__((weak));
 
 int gk_problem_kind = -1;
 uint64_t gk_problem_seed = 0;
@@ -104,10 +122,13 @@ static void gk_draw_parameters(void) {
   p_b0[0] = sz * cos(az);
   p_b0[1] = sz * sin(az);
   p_b0[2] = cz;
-  if (gk_problem_kind == GK_DIPOLE_SHELL) {
+  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_problem_kind == GK_POLE_SPHERE) {
     p_m = 0.5 + 1.5 * gk_scalar(13);
     p_ctr[0] = 3.0 + 4.0 * gk_scalar(14);               /* r */
-    p_ctr[1] = PI * (0.375 + 0.25 * gk_scalar(15));     /* theta */
+    if (gk_problem_kind == GK_POLE_SPHERE)
+      p_ctr[1] = PI * (0.05 + 0.1 * gk_scalar(15));     /* theta, near the pole */
+    else
+      p_ctr[1] = PI * (0.375 + 0.25 * gk_scalar(15));   /* theta */
     p_ctr[2] = twopi * gk_scalar(16);                   /* phi */
   } else {
     for (int n = 0; n < 3; n++) p_ctr[n] = gk_scalar(10 + n);
@@ -148,12 +169,12 @@ void set_problem_config() {
   gk_resolve_selection();
   gk_draw_parameters();
   CA = 10.0;
-  doRingAverage = 0;
-  if (gk_problem_kind == GK_DIPOLE_SHELL) {
+  doRingAverage = gk_problem_kind == GK_POLE_SPHERE ? 1 : 0;
+  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_problem_kind == GK_POLE_SPHERE) {
     x1min_global = 1.0;
     x1max_global = 10.0;
-    x2min_global = PI / 8.0;
-    x2max_global = 7.0 * PI / 8.0;
+    x2min_global = gk_problem_kind == GK_POLE_SPHERE ? 0.0 : PI / 8.0;
+    x2max_global = gk_problem_kind == GK_POLE_SPHERE ? PI : 7.0 * PI / 8.0;
     x3min_global = 0.0;
     x3max_global = 2.0 * PI;
     problem_config.Boundary_i = BC_NON_PERIODIC;
@@ -186,7 +207,7 @@ void set_problem_config() {
  * the geometry arrays.
  */
 int gk_problem_custom_grid(void) {
-  if (gk_problem_kind != GK_DIPOLE_SHELL) return 0;
+  if (gk_problem_kind != GK_DIPOLE_SHELL && gk_problem_kind != GK_POLE_SPHERE) return 0;
   for (int i = isg; i <= ieg + 1; i++) {
     int gi = i - is + proc_coords[0] * config.ni;
     double r = pow(10.0, (double)gi / (double)config.ni_global);
@@ -206,6 +227,15 @@ void boundary_conditions() {
     gem_bc_symmetric_i_high();
     gem_bc_symmetric_j_low();
     gem_bc_symmetric_j_high();
+  } else if (gk_problem_kind == GK_POLE_SPHERE && gas_bc_pole_j_low) {
+    gas_bc_symmetric_i_low();
+    gas_bc_symmetric_i_high();
+    gas_bc_pole_j_low();
+    gas_bc_pole_j_high();
+    gem_bc_symmetric_i_low();
+    gem_bc_symmetric_i_high();
+    gem_bc_pole_j_low();
+    gem_bc_pole_j_high();
   }
 }
 
@@ -268,6 +298,44 @@ static void gk_init_dipole_shell(void) {
          Field_Default, Field_Default);
 }
 
+static void gk_init_pole_sphere(void) {
+  const double twopi = 2.0 * PI;
+  double xc = p_ctr[0] * sin(p_ctr[1]) * cos(p_ctr[2]);
+  double yc = p_ctr[0] * sin(p_ctr[1]) * sin(p_ctr[2]);
+  double zc = p_ctr[0] * cos(p_ctr[1]);
+  /* flow direction: isotropic unit vector of the seeded draws 8, 9 */
+  double cz = 2.0 * gk_scalar(8) - 1.0;
+  double az = twopi * gk_scalar(9);
+  double sz = sqrt(fmax(0.0, 1.0 - cz * cz));
+  double u[3] = {0.3 * sz * cos(az), 0.3 * sz * sin(az), 0.3 * cz};
+  for (int s = 0; s < NS1; s++) {
+    for (int i = is; i <= ie; i++) {
+      for (int j = js; j <= je; j++) {
+        for (int k = ks; k <= ke; k++) {
+          int64_t gi = i - is + proc_coords[0] * config.ni;
+          int64_t gj = j - js + proc_coords[1] * config.nj;
+          int64_t gk = k - ks + proc_coords[2] * config.nk;
+          double xi = 2.0 * gk_hash_u01(gk_problem_seed, 100, gi, gj, gk) - 1.0;
+          double r = x1ctr[i][j][k], th = x2ctr[i][j][k], ph = x3ctr[i][j][k];
+          double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+          double x = r * st * cp - xc;
+          double y = r * st * sp - yc;
+          double z = r * ct - zc;
+          double sm = gk_ramp(sqrt(x * x + y * y + z * z), 0.5, dx1[i][j][k]);
+          gas[s][gas_rho][i][j][k] = 1.0 + 0.1 * xi;
+          gas[s][gas_v1][i][j][k] = u[0] * st * cp + u[1] * st * sp + u[2] * ct;
+          gas[s][gas_v2][i][j][k] = u[0] * ct * cp + u[1] * ct * sp - u[2] * st;
+          gas[s][gas_v3][i][j][k] = -u[0] * sp + u[1] * cp;
+          gas[s][gas_p][i][j][k] = 0.1 * (1.0 + 49.0 * sm);
+        }
+      }
+    }
+  }
+  A3 = gk_Aphi_dipole;
+  AinitB(gem[mag_bi], gem[mag_bj], gem[mag_bk], A1, A2, A3, Field_Default,
+         Field_Default, Field_Default);
+}
+
 void problem_init() {
   log_info("Initializing seeded problem kind %d seed %llu", gk_problem_kind,
            (unsigned long long)gk_problem_seed);
@@ -283,6 +351,9 @@ void problem_init() {
       break;
     case GK_DIPOLE_SHELL:
       gk_init_dipole_shell();
+      break;
+    case GK_POLE_SPHERE:
+      gk_init_pole_sphere();
       break;
     default:
       log_error("unknown seeded problem kind %d", gk_problem_kind);

@@ -34,8 +34,13 @@ typedef enum {
   GK_ERR_NAN = -5,      /* non-finite state detected by the fused reduction */
 } gk_status;
 
-/* boundary kinds per array, dimension and side */
-enum { GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2 };
+/* boundary kinds per array, dimension and side.  GK_BC_POLE (j sides of a
+   spherical grid whose theta range ends at a pole; replaces gas_bc_pole_j_* /
+   gem_bc_pole_j_*, src/spherical/spherical_boundary.c:23-138): each ghost row
+   takes the row mirrored across the pole, rotated by pi in phi
+   (k -> 4 + (k - 4 + nk/2) mod nk), for every i and k of the box, with v_theta,
+   v_phi and B_theta, B_phi negated.  It needs ring averaging (below). */
+enum { GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2, GK_BC_POLE = 3 };
 
 /* separable geometry quantities: q(i,j,k) = f[q][0][i] * f[q][1][j] * f[q][2][k] */
 enum {
@@ -77,6 +82,17 @@ typedef struct {
   double b0_face[3][8];    /* per face direction: b1 b2 b3 bn bsq bn1 bn2 bn3 */
   double b0_edge[3][3];    /* per edge direction: b1 b2 b3 */
   double b0_lorentz[3];    /* dmagsf_G, constant */
+  /* pole ring averaging (doRingAverage; src/spherical/spherical_RingAverage.c,
+     called from src/solver_solve.c:258-260,308-310,486-491): ring = 1 turns it
+     on (spherical, FP64, k periodic).  ring_pole[0|1]: theta = 0 / theta = pi
+     is a pole of the grid; the ring_rows cell rows next to each pole are
+     averaged, row r (0 at the pole) over ring_nchunk[r] azimuthal blocks, and
+     the CFL time step uses the phi width times nk / ring_nchunk[r] on them.
+     Zero-initialise the descriptor: ring = 0 leaves all of this unused. */
+  int ring;
+  int ring_pole[2];
+  int ring_rows;
+  int ring_nchunk[16];
 } gk_grid_desc;
 
 typedef struct gk_solver gk_solver;

@@ -81,6 +81,7 @@ struct go_solver {
   double *crd[12]; /* x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
   double dt, dt0, time_sim, divB_max;
   double *dbgF; /* optional per-direction copies of the 9 face fluxes */
+  int ring, pole[2], navr, nchunk[16]; /* pole ring averaging */
 };
 
 #define IDX(i, j, k) ((((int64_t)(i)) * s->NJ + (j)) * s->NK + (k))
@@ -240,14 +241,39 @@ static void bc_symmetric(go_solver *s, double *base, int dim, int side,
   }
 }
 
+/* pole boundary for one j side of fields 0..5 (src/spherical/
+   spherical_boundary.c:23-138): the ghost rows take the row mirrored across
+   the pole, rotated by pi in phi (k -> NG + (k - NG + nk/2) mod nk), for every
+   i and k of the box; v2, v3 (gas) and B_j, B_k, b2, b3 (gem) change sign. */
+static void bc_pole(go_solver *s, double *base, int side, int is_gem) {
+  for (int f = 0; f < 6; f++) {
+    double *a = &base[(int64_t)f * s->n3];
+    int o = (is_gem && f == 1) ? 1 : 0;
+    double sgn = is_gem ? ((f == 0 || f == 3) ? 1.0 : -1.0)
+                        : ((f == 2 || f == 3) ? -1.0 : 1.0);
+    for (int i = 0; i < s->NI; i++)
+      for (int g = 0; g < NG; g++) {
+        int dst = side == 0 ? g : s->NJ - 1 - NG + g + o;
+        int src = side == 0 ? 2 * NG - 1 - g + o : s->NJ - 1 - NG - 1 - g;
+        for (int k = 0; k < s->NK; k++) {
+          int kr = NG + (k - NG + s->nk / 2) % s->nk;
+          a[IDX(i, dst, k)] = sgn * a[IDX(i, src, kr)];
+        }
+      }
+  }
+}
+
 /* boundary_conditions(): gas i low/high, j low/high, k low/high, then gem in
    the same order -- the order the shipped plugins call the BC library. */
 static void boundary_conditions(go_solver *s) {
   for (int arr = 0; arr < 2; arr++)
     for (int dim = 0; dim < 3; dim++)
-      for (int side = 0; side < 2; side++)
+      for (int side = 0; side < 2; side++) {
         if (s->bc[2 * dim + side] == GO_BC_SYMMETRIC)
           bc_symmetric(s, arr ? s->gem : s->gas, dim, side, arr);
+        if (s->bc[2 * dim + side] == GO_BC_POLE && dim == 1)
+          bc_pole(s, arr ? s->gem : s->gas, side, arr);
+      }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -323,8 +349,11 @@ static double get_dt(go_solver *s) {
         double Vfluid = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
         double Vsound = sqrt(gamma_val * GAS(gas_p, i, j, k) / rho);
         double VCFL = Vfluid + sqrt(Vsound * Vsound + Va_eff * Va_eff);
+        /* get_dt_ring (src/spherical/spherical_general.c:700-735) differs
+           only in the azimuthal width */
+        double dxk3 = s->ring ? GEO(dxk_ring, i, j, k) : GEO(dxk, i, j, k);
         double dtCFL = CFL / (VCFL / GEO(dxi, i, j, k) + VCFL / GEO(dxj, i, j, k) +
-                              VCFL / GEO(dxk, i, j, k));
+                              VCFL / dxk3);
         m = fmin(m, dtCFL);
       }
   return m;
@@ -721,6 +750,267 @@ static void flux_divergence(go_solver *s, int dir) {
 }
 
 /* one pass of the solve() loop body (src/solver_solve.c:103-508) */
+
+/* ---------------------------------------------------------------------- */
+/* Pole ring averaging (src/spherical/spherical_RingAverage.c, helpers
+   src/solver_func.c:2136-2257).  Single rank: the gather / broadcast copies
+   (src/spherical/spherical_mpi.c) are the identity, so the routines work on
+   gas / gem in place; the ring E fields live in a scratch block. */
+
+/* blocks per ring row by azimuthal resolution
+   (src/spherical/spherical_RingAverage.c:55-79) */
+int go_set_ring(go_solver *s, int on, int pole_lo, int pole_hi) {
+  static const int t256[16] = {8, 16, 32, 32, 64, 64, 64, 64, 128, 128, 128, 128, 128, 128, 128, 128};
+  s->ring = on;
+  s->pole[0] = on && pole_lo;
+  s->pole[1] = on && pole_hi;
+  if (!on) return 0;
+  int n;
+  if (s->nk == 256) n = 16;
+  else if (s->nk == 128) n = 8;
+  else if (s->nk == 64) n = 4;
+  else if (s->nk == 32) n = 2;
+  else return -1;
+  s->navr = n;
+  for (int r = 0; r < n; r++) s->nchunk[r] = t256[r];
+  /* geo[dxk_ring] (src/spherical/spherical_RingAverage.c:81-147) */
+  for (int i = s->isg; i <= s->ieg; i++)
+    for (int j = s->jsg; j <= s->jeg; j++)
+      for (int k = s->ksg; k <= s->keg; k++) GEO(dxk_ring, i, j, k) = GEO(dxk, i, j, k);
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    for (int r = 0; r < n; r++) {
+      int Nm = s->nk / s->nchunk[r];
+      int j = p == 0 ? s->js + r : s->je - r;
+      for (int i = s->is; i <= s->ie; i++)
+        for (int k = s->ks; k <= s->ke; k++) GEO(dxk_ring, i, j, k) = Nm * GEO(dxk, i, j, k);
+    }
+  }
+  return n;
+}
+
+/* PPM (src/solver_func.c:2148-2185) with minmod (:2139-2141) */
+static double ring_minmod(double x, double y) {
+  return 0.5 * (fsign(x) + fsign(y)) * fmin(fabs(x), fabs(y));
+}
+static void ring_ppm(double fm2, double fm1, double f, double fp1, double fp2, double *vm,
+                     double *vp) {
+  double dvm2 = fm1 - fm2, dvm1 = f - fm1, dvp1 = fp1 - f, dvp2 = fp2 - fp1;
+  double dvc = 0.5 * (dvm2 + dvm1);
+  double SM = 2.0 * ring_minmod(dvm2, dvm1);
+  double Sm1 = ring_minmod(dvc, SM);
+  dvc = 0.5 * (dvm1 + dvp1);
+  SM = 2.0 * ring_minmod(dvm1, dvp1);
+  double Sp1 = ring_minmod(dvc, SM);
+  dvc = 0.5 * (dvp2 + dvp1);
+  SM = 2.0 * ring_minmod(dvp2, dvp1);
+  double Sp2 = ring_minmod(dvc, SM);
+  *vp = 0.5 * (f + fp1) - (Sp2 - Sp1) / 6.0;
+  *vm = 0.5 * (f + fm1) - (Sp1 - Sm1) / 6.0;
+  double ap = *vp - f, am = *vm - f;
+  if (ap * am >= 0.0) {
+    ap = 0.0;
+    am = 0.0;
+  } else {
+    if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+    if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+  }
+  *vp = f + ap;
+  *vm = f + am;
+}
+
+/* FilterBaseMode (src/solver_func.c:2187-2212): Q0 := the m = 0, 1 Fourier
+   part of Q, Q -= Q0 */
+static void ring_filter(double *Q, int N, double *Q0) {
+  const double scl = 2.0 / N, dtheta = 2.0 * 3.14159265358979323846 / N;
+  double a0 = 0.0, a1 = 0.0, b1 = 0.0;
+  for (int i = 0; i < N; i++) {
+    double th = dtheta * (i + 0.5);
+    a0 += Q[i];
+    a1 += Q[i] * cos(th);
+    b1 += Q[i] * sin(th);
+  }
+  a1 *= scl;
+  b1 *= scl;
+  a0 *= 0.5 * scl;
+  for (int i = 0; i < N; i++) {
+    double th = dtheta * (i + 0.5);
+    Q0[i] = a0 + a1 * cos(th) + b1 * sin(th);
+    Q[i] -= Q0[i];
+  }
+}
+
+/* RingBlockAvg (src/solver_func.c:2214-2257): block means of nch blocks,
+   periodic PPM edge values per block, parabola integrated over each cell */
+static void ring_block_avg(double *Q, int N, int nch) {
+  int ncell = N / nch;
+  double blk[128], ext[132];
+  for (int ib = 0; ib < nch; ib++) {
+    double sum = 0.0;
+    for (int ic = 0; ic < ncell; ic++) sum += Q[ib * ncell + ic];
+    blk[ib] = sum / ncell;
+  }
+  ext[0] = blk[nch - 2];
+  ext[1] = blk[nch - 1];
+  for (int ib = 0; ib < nch; ib++) ext[ib + 2] = blk[ib];
+  ext[nch + 2] = blk[0];
+  ext[nch + 3] = blk[1];
+  for (int ib = 2; ib < nch + 2; ib++) {
+    int ig = (ib - 2) * ncell;
+    double fL, fR;
+    ring_ppm(ext[ib - 2], ext[ib - 1], ext[ib], ext[ib + 1], ext[ib + 2], &fL, &fR);
+    double a = 3.0 * (fL + fR - 2.0 * ext[ib]);
+    double b = 2.0 * (3.0 * ext[ib] - fR - 2.0 * fL);
+    double c = fL;
+    for (int ic = 0; ic < ncell; ic++) {
+      double kl = (double)ic + 1.0;
+      Q[ig + ic] = (a / 3.0 / ncell / ncell) * (3 * kl * kl - 3 * kl + 1) + 0.5 * b * (2 * kl - 1) / ncell + c;
+    }
+  }
+}
+
+void go_ring_filter(double *Q, int N, double *Q0) { ring_filter(Q, N, Q0); }
+void go_ring_block_avg(double *Q, int N, int nch) { ring_block_avg(Q, N, nch); }
+
+/* tackle_Efield_pole (src/spherical/spherical_RingAverage.c:153-201):
+   E_i on the pole edges := its ring mean over nk + 1 edges; E_k := 0 */
+static void ring_efield_pole(go_solver *s) {
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    int jp = p == 0 ? s->js : s->je + 1;
+    for (int i = s->is; i <= s->ie; i++) {
+      double m = 0.0;
+      for (int k = s->ks; k <= s->ke + 1; k++) m = m + GEM(efield_ei, i, jp, k);
+      m = m / (s->nk + 1);
+      for (int k = s->ks; k <= s->ke + 1; k++) GEM(efield_ei, i, jp, k) = m;
+    }
+    for (int i = s->is; i <= s->ie + 1; i++)
+      for (int k = s->ks; k <= s->ke; k++) GEM(efield_ek, i, jp, k) = 0.0;
+  }
+}
+
+/* tackle_Magfield_pole (src/spherical/spherical_RingAverage.c:203-235):
+   B_j on the pole face from the next face row and its antipode in phi */
+static void ring_magfield_pole(go_solver *s) {
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    int jp = p == 0 ? s->js : s->je + 1, jn = p == 0 ? s->js + 1 : s->je;
+    for (int i = s->is; i <= s->ie; i++)
+      for (int k = s->ks; k <= s->ke; k++)
+        GEM(mag_bj, i, jp, k) = 0.5 * (GEM(mag_bj, i, jn, k) -
+                                       GEM(mag_bj, i, jn, NG + (k - NG + s->nk / 2) % s->nk));
+  }
+}
+
+/* HydroRingAverage_base (src/spherical/spherical_RingAverage.c:237-377):
+   rho and p block-averaged; each velocity component keeps its m = 0, 1 part
+   and block-averages the rest */
+static void ring_hydro(go_solver *s) {
+  const int N = s->nk;
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+#pragma omp parallel for schedule(static)
+    for (int i = s->is; i <= s->ie; i++) {
+      double Q[256], base[256], rem[256];
+      for (int r = 0; r < s->navr; r++) {
+        int j = p == 0 ? s->js + r : s->je - r;
+        for (int f = gas_rho; f <= gas_p; f++) {
+          for (int k = 0; k < N; k++) Q[k] = GAS(f, i, j, s->ks + k);
+          if (f == gas_rho || f == gas_p) {
+            ring_block_avg(Q, N, s->nchunk[r]);
+          } else {
+            ring_filter(Q, N, base);
+            memcpy(rem, Q, sizeof(double) * N);
+            ring_block_avg(rem, N, s->nchunk[r]);
+            for (int k = 0; k < N; k++) Q[k] = rem[k] + base[k];
+          }
+          for (int k = 0; k < N; k++) GAS(f, i, j, s->ks + k) = Q[k];
+        }
+      }
+    }
+  }
+}
+
+/* MagneticRingAverage_base (src/spherical/spherical_RingAverage.c:380-677):
+   the face-B flux through each ring row is block-averaged (m = 0, 1 kept);
+   the change is written as edge E fields inside each block (cumulative sums
+   of the flux differences) and applied by a curl update, so div B is
+   unchanged; then the pole-face B_j copy and the zero-mean pole-row B_k. */
+static void ring_magnetic(go_solver *s, double *E) {
+  const int N = s->nk;
+  double *Ei = E, *Ej = E + s->n3;
+  memset(E, 0, sizeof(double) * 2 * (size_t)s->n3);
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    for (int comp = 0; comp < 2; comp++) {
+      /* comp 0: B_j flux on face rows next to the cell rows, E_i (negated);
+         comp 1: B_i flux on the cell rows, E_j */
+      int i1 = comp == 0 ? s->ie : s->ie + 1;
+#pragma omp parallel for schedule(static)
+      for (int i = s->is; i <= i1; i++) {
+        double fl[256], base[256], rem[256];
+        for (int r = 0; r < s->navr; r++) {
+          int j = comp == 0 ? (p == 0 ? s->js + r + 1 : s->je - r) : (p == 0 ? s->js + r : s->je - r);
+          for (int k = 0; k < N; k++)
+            fl[k] = comp == 0 ? GEM(mag_bj, i, j, s->ks + k) * GEO(face_jdir, i, j, s->ks + k)
+                              : GEM(mag_bi, i, j, s->ks + k) * GEO(face_idir, i, j, s->ks + k);
+          ring_filter(fl, N, base);
+          memcpy(rem, fl, sizeof(double) * N);
+          ring_block_avg(rem, N, s->nchunk[r]);
+          int ncell = N / s->nchunk[r];
+          for (int blk = 0; blk < s->nchunk[r]; blk++) {
+            int start = blk * ncell;
+            double sum = 0.0;
+            for (int m = 0; m < ncell - 1; m++) {
+              sum += rem[start + m] - fl[start + m];
+              if (comp == 0) Ei[IDX(i, j, start + s->ks + 1 + m)] = -sum;
+              else Ej[IDX(i, j, start + s->ks + 1 + m)] = sum;
+            }
+          }
+        }
+      }
+    }
+  }
+  /* curl update over the active faces (E_k is zero) */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = s->is; i <= s->ie + 1; i++)
+    for (int j = s->js; j <= s->je; j++)
+      for (int k = s->ks; k <= s->ke; k++) {
+        double c = 1.0 * ((0.0 - 0.0) - (Ej[IDX(i, j, k + 1)] - Ej[IDX(i, j, k)]));
+        GEM(mag_bi, i, j, k) -= c / GEO(face_idir, i, j, k);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = s->is; i <= s->ie; i++)
+    for (int j = s->js; j <= s->je + 1; j++)
+      for (int k = s->ks; k <= s->ke; k++) {
+        double c = 1.0 * ((Ei[IDX(i, j, k + 1)] - Ei[IDX(i, j, k)]) - (0.0 - 0.0));
+        GEM(mag_bj, i, j, k) -= c / GEO(face_jdir, i, j, k);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = s->is; i <= s->ie; i++)
+    for (int j = s->js; j <= s->je; j++)
+      for (int k = s->ks; k <= s->ke + 1; k++) {
+        double c = 1.0 * ((Ej[IDX(i + 1, j, k)] - Ej[IDX(i, j, k)]) - (Ei[IDX(i, j + 1, k)] - Ei[IDX(i, j, k)]));
+        GEM(mag_bk, i, j, k) -= c / GEO(face_kdir, i, j, k);
+      }
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    int jp = p == 0 ? s->js : s->je + 1, jn = p == 0 ? s->js + 1 : s->je;
+    for (int i = s->is; i <= s->ie; i++)
+      for (int k = s->ks; k <= s->ke; k++) GEM(mag_bj, i, jp, k) = GEM(mag_bj, i, jn, k);
+  }
+  for (int p = 0; p < 2; p++) {
+    if (!s->pole[p]) continue;
+    int jr = p == 0 ? s->js : s->je;
+    for (int i = s->is; i <= s->ie; i++) {
+      double m = 0.0;
+      for (int k = s->ks; k <= s->ke + 1; k++) m += GEM(mag_bk, i, jr, k);
+      m = m / (s->nk + 1);
+      for (int k = s->ks; k <= s->ke + 1; k++) GEM(mag_bk, i, jr, k) -= m;
+    }
+  }
+}
+
 static void step_once(go_solver *s) {
   const double CA = s->CA;
   double dt = get_dt(s);
@@ -767,6 +1057,8 @@ static void step_once(go_solver *s) {
     flux_divergence(s, dir);
   }
 
+  if (s->ring) ring_efield_pole(s); /* src/solver_solve.c:258-260 */
+
   /* CT update (src/solver_solve.c:263-306) */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int i = s->is; i <= s->ie + 1; i++)
@@ -795,6 +1087,8 @@ static void step_once(go_solver *s) {
                                       (GEM(efield_ei, i, j + 1, k) * GEO(edge_idir, i, j + 1, k) -
                                        GEM(efield_ei, i, j, k) * GEO(edge_idir, i, j, k))) /
                                 GEO(face_kdir, i, j, k);
+
+  if (s->ring) ring_magfield_pole(s); /* src/solver_solve.c:308-310 */
 
   /* B0 Lorentz add, conservative update, cons->prim, positivity, cell B,
      magtot and the Boris momentum update (src/solver_solve.c:314-484,
@@ -863,6 +1157,13 @@ static void step_once(go_solver *s) {
         for (int m = 0; m < 3; m++)
           GAS(gas_v1 + m, i, j, k) = GAS(gasc_rhov1 + m, i, j, k) / GAS(gasc_rho, i, j, k);
       }
+
+  if (s->ring) { /* src/solver_solve.c:486-491 */
+    ring_hydro(s);
+    double *E = (double *)malloc(sizeof(double) * 2 * (size_t)s->n3);
+    if (E) ring_magnetic(s, E);
+    free(E);
+  }
 
   boundary_exchange(s);
   boundary_conditions(s);

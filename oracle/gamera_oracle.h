@@ -35,8 +35,9 @@ enum {
   GO_DX1, GO_DX2, GO_DX3,
 };
 
-/* boundary kinds per (dimension, side) */
-enum { GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2 };
+/* boundary kinds per (dimension, side); GO_BC_POLE only on the j sides of
+   a spherical grid whose theta range ends at a pole */
+enum { GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2, GO_BC_POLE = 3 };
 
 /* coord: 0 Cartesian, 2 spherical (selects the geometric source terms). */
 go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],
@@ -44,6 +45,13 @@ go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],
 void go_destroy(go_solver *s);
 double *go_array(go_solver *s, int which);
 int64_t go_npoints(const go_solver *s);
+
+/* Pole ring averaging (src/spherical/spherical_RingAverage.c): on = 1 turns
+   it on (doRingAverage); pole_lo / pole_hi: theta = 0 / theta = pi is a pole
+   of the grid.  Call after the geometry is loaded and before go_prologue():
+   it sets geo[dxk_ring] as set_RingAverage_config does.  Returns the number
+   of averaged rows per pole (NAVR), or -1 if nk is not 32, 64, 128 or 256. */
+int go_set_ring(go_solver *s, int on, int pole_lo, int pole_hi);
 
 /* solve() prologue (solver_solve.c:25-99): ghost fill, BCs, derived
    variables, X_p := X, dt0 := get_dt(). */
@@ -64,6 +72,12 @@ double go_pdmu7(const double fh0[7], const double fp0[7], const double w[7], dou
                 double facei, double vol, double dx);
 void go_face_flux(int dir, const double L[6], const double R[6], const double bl[3],
                   const double br[3], const double b0[8], double Bn, double CA, double out[9]);
+
+/* unit-test hooks of the ring average: FilterBaseMode (Q -= its m = 0, 1
+   Fourier part Q0; src/solver_func.c:2187-2212) and RingBlockAvg
+   (src/solver_func.c:2214-2257), N <= 256, nch <= 128 */
+void go_ring_filter(double *Q, int N, double *Q0);
+void go_ring_block_avg(double *Q, int N, int nch);
 
 /* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max */
 double go_scalar(const go_solver *s, int which);

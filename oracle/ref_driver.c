@@ -5,7 +5,8 @@
  * src/main.c) plus a problem plugin; it replaces src/main.c:14-99 with the
  * same call sequence (MPI init, set_problem_config, initialize_config,
  * setup_comm_and_types, initialize_solver, getweights, problem_init,
- * set_background_field, solve(Nt)) and then writes the solver's global arrays
+ * set_background_field, the ring-average set-up when the plugin turns it on,
+ * solve(Nt)) and then writes the solver's global arrays
  * as raw little-endian float64 records so tests can read them with numpy.
  *
  * Usage: ref_driver <config.yaml> <out.bin> <dump_geometry 0|1>
@@ -63,6 +64,10 @@ int main(int argc, char *argv[]) {
   getweights();
   if (!read_restart) problem_init();
   set_background_field();
+  if (doRingAverage == 1) { /* src/main.c:66-70 */
+    set_RingAverage_geo_data();
+    set_RingAverage_config();
+  }
   dtout = 10.0;
   if (solve(Nt) != 0) return 6;
 

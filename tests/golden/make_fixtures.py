@@ -30,8 +30,12 @@ STATE_CASES = [
     (1, 3, 16, 16, 16, 20),
     (2, 4, 16, 12, 16, 20),
     (3, 5, 24, 16, 8, 40),
+    # full sphere with pole ring averaging (2 and 4 averaged rows per pole)
+    (4, 8, 12, 16, 32, 12),
+    (4, 9, 8, 12, 64, 8),
 ]
-GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12)]
+GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
+              (4, 8, 12, 16, 32)]
 CONFIG0 = (0, 1, 128, 128, 8, 200)
 
 COORDS = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr", "dx1", "dx2", "dx3"]

@@ -16,7 +16,7 @@ NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
 GO_GAS, GO_GEM, GO_GEO, GO_REC = 0, 1, 2, 3
 COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr",
                "x3ctr", "dx1", "dx2", "dx3"]
-BC_PERIODIC, BC_SYMMETRIC, BC_NONE = 0, 1, 2
+BC_PERIODIC, BC_SYMMETRIC, BC_NONE, BC_POLE = 0, 1, 2, 3
 
 _libs = {}
 
@@ -50,6 +50,8 @@ def load(fast=False):
     lib.go_face_flux.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 5 + [
         ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     lib.go_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    lib.go_set_ring.restype = ctypes.c_int
+    lib.go_set_ring.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     _libs[name] = lib
     return lib
 
@@ -94,6 +96,14 @@ class Oracle:
         p = self.lib.go_debug_fluxes(self.h)
         return np.ctypeslib.as_array(p, shape=(45,) + self.shape)
 
+    def set_ring(self, on, pole_lo, pole_hi):
+        """Turn pole ring averaging on (after the geometry is loaded); returns
+        the number of averaged rows per pole."""
+        n = self.lib.go_set_ring(self.h, int(on), int(pole_lo), int(pole_hi))
+        if n < 0:
+            raise ValueError("ring averaging needs nk in 32, 64, 128, 256")
+        return n
+
     def prologue(self):
         self.lib.go_prologue(self.h)
 
@@ -113,6 +123,27 @@ class Oracle:
 
     def __del__(self):
         self.close()
+
+
+def ring_filter(q):
+    """FilterBaseMode on a copy of q: returns (remainder, m = 0, 1 part)."""
+    lib = load()
+    q = np.ascontiguousarray(q, dtype=np.float64).copy()
+    base = np.zeros_like(q)
+    P = ctypes.POINTER(ctypes.c_double)
+    lib.go_ring_filter.argtypes = [P, ctypes.c_int, P]
+    lib.go_ring_filter(q.ctypes.data_as(P), len(q), base.ctypes.data_as(P))
+    return q, base
+
+
+def ring_block_avg(q, nch):
+    """RingBlockAvg on a copy of q with nch blocks."""
+    lib = load()
+    q = np.ascontiguousarray(q, dtype=np.float64).copy()
+    P = ctypes.POINTER(ctypes.c_double)
+    lib.go_ring_block_avg.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    lib.go_ring_block_avg(q.ctypes.data_as(P), len(q), nch)
+    return q
 
 
 def pdmu7(fh, fp, w, faceim1, facei, vol, dx):

@@ -14,13 +14,13 @@ def oracle_from_problem(P, fast=False):
     ni, nj, nk = P.n
     bc = P.bc_table()
     kinds = []
-    periodic = [P.coord == 0, P.coord == 0, True] if P.kind != 2 else [False, False, True]
+    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4) else [False, False, True]
     for dim in range(3):
         for side in range(2):
             if periodic[dim]:
                 kinds.append(0)
             else:
-                kinds.append(1 if bc[0][2 * dim + side] == 1 else 2)
+                kinds.append({1: 1, 6: 3}.get(bc[0][2 * dim + side], 2))
     O = Oracle(ni, nj, nk, coord=P.coord, bc=tuple(kinds), CA=P.CA, fast=fast)
     for f in range(28):
         O.geo[f] = P.geo(f)
@@ -34,6 +34,8 @@ def oracle_from_problem(P, fast=False):
         O.gas[f] = P.gas(f)
     for f in range(3):
         O.gem[f] = P.gem(f)
+    if P.kind == 4:  # full sphere: ring averaging at both poles
+        O.set_ring(1, 1, 1)
     return O
 
 

@@ -68,6 +68,11 @@ def test_boundary_kinds_recorded_from_plugin():
         assert bc.tolist() == [[1, 1, 1, 1, 0, 0], [1, 1, 1, 1, 0, 0]]
     finally:
         P.close()
+    P = go.Problem(go.POLE_SPHERE, 1, 12, 12, 32)
+    try:  # 6 = pole boundary on both theta sides
+        assert P.bc_table().tolist() == [[1, 1, 6, 6, 0, 0], [1, 1, 6, 6, 0, 0]]
+    finally:
+        P.close()
     P = go.Problem(go.SINUSOID, 1, 12, 10, 8)
     try:
         assert P.bc_table().tolist() == [[0] * 6, [0] * 6]

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import gamera_op as go
-from oracle_lib import face_flux, pdmu7
+from oracle_lib import face_flux, pdmu7, ring_block_avg, ring_filter
 from parity import active_slices, oracle_from_problem, run_oracle
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -20,7 +20,9 @@ STATES = np.load(os.path.join(GOLD, "ref_states.npz"))
 HASHES = json.load(open(os.path.join(GOLD, "ref_hashes.json")))
 
 STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16, 20),
-               (2, 4, 16, 12, 16, 20), (3, 5, 24, 16, 8, 40)]
+               (2, 4, 16, 12, 16, 20), (3, 5, 24, 16, 8, 40),
+               # full sphere, pole ring averaging on (2 and 4 rows per pole)
+               (4, 8, 12, 16, 32, 12), (4, 9, 8, 12, 64, 8)]
 
 
 def sha(a):
@@ -172,3 +174,47 @@ def test_mass_conserved_and_divb_roundoff_periodic():
         assert O.scalar("divB_max") < 1e-12
     finally:
         P.close()
+
+
+# --- ring-average helpers (pole rows of full-sphere grids) ----------------
+@pytest.mark.parametrize("N", [32, 64, 256])
+def test_ring_filter_splits_off_modes_0_and_1(N):
+    """FilterBaseMode: a signal made of modes 0 and 1 goes entirely into the
+    base part; higher modes entirely into the remainder."""
+    th = 2.0 * np.pi / N * (np.arange(N) + 0.5)
+    low = 0.7 - 1.3 * np.cos(th) + 0.4 * np.sin(th)
+    high = 0.2 * np.cos(3 * th) - 0.5 * np.sin(5 * th) + 0.1 * np.cos((N // 2) * th)
+    rem, base = ring_filter(low + high)
+    assert np.allclose(base, low, rtol=0, atol=1e-13)
+    assert np.allclose(rem, high, rtol=0, atol=1e-13)
+    rem0, base0 = ring_filter(low)
+    assert np.max(np.abs(rem0)) < 1e-13
+
+
+@pytest.mark.parametrize("N,nch", [(32, 8), (32, 16), (64, 32), (256, 128), (256, 64)])
+def test_ring_block_avg_conserves_block_means(N, nch):
+    """RingBlockAvg: each block keeps its mean (the parabola's cell averages
+    integrate to it), a constant is a fixed point, and the result is a
+    function of the block means only."""
+    rng = np.random.default_rng(N + nch)
+    q = rng.standard_normal(N)
+    out = ring_block_avg(q, nch)
+    nc = N // nch
+    assert np.allclose(out.reshape(nch, nc).sum(1), q.reshape(nch, nc).sum(1), rtol=0, atol=1e-12)
+    c = np.full(N, 2.5)
+    assert np.array_equal(ring_block_avg(c, nch), c)
+    # shuffling values inside each block leaves the output unchanged
+    q2 = q.reshape(nch, nc)[:, ::-1].reshape(-1)
+    assert np.allclose(ring_block_avg(q2, nch), out, rtol=0, atol=1e-13)
+
+
+def test_ring_block_avg_smooth_profile_within_limiter():
+    """A smooth periodic profile sampled finely is reproduced to the accuracy
+    of a parabolic reconstruction, and no new extrema appear (the PPM edge
+    values are limited)."""
+    N, nch = 256, 64
+    th = 2.0 * np.pi / N * (np.arange(N) + 0.5)
+    q = np.sin(th) + 0.3 * np.cos(2 * th)
+    out = ring_block_avg(q, nch)
+    assert np.max(np.abs(out - q)) < 5e-3
+    assert out.max() <= q.max() + 1e-12 and out.min() >= q.min() - 1e-12
