@@ -1629,6 +1629,7 @@ struct DtArgs {
   Geo geo;
   B0c b0;
   const double *X, *M, *B;
+  const double *dxkj;  // j factor of the phi width in the CFL dt
   Scal *sc;
 };
 
@@ -1644,9 +1645,12 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
     double b[3];
 #pragma unroll
     for (int m = 0; m < 3; m++) b[m] = a.M[m * g.fs + c] + a.b0.cell[m];
+    // phi width: the ring-widened one on averaged rows (get_dt_ring,
+    // src/spherical/spherical_general.c:700-735); the j factor carries it
+    const double dxk = G.g[GK_G_DXK][0][i] * a.dxkj[j] * G.g[GK_G_DXK][2][k];
     dtc = cfl_dt(a.X[c], a.X[g.fs + c], a.X[2 * g.fs + c], a.X[3 * g.fs + c], a.X[4 * g.fs + c],
-                 b[0], b[1], b[2], geo3(G, GK_G_DXI, i, j, k), geo3(G, GK_G_DXJ, i, j, k),
-                 geo3(G, GK_G_DXK, i, j, k), g.CA);
+                 b[0], b[1], b[2], geo3(G, GK_G_DXI, i, j, k), geo3(G, GK_G_DXJ, i, j, k), dxk,
+                 g.CA);
     if (!(dtc == dtc)) dtc = INFINITY;
     const double V = geo3(G, GK_G_VOL, i, j, k);
     const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
@@ -1666,7 +1670,8 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
 // Ghost fill.  Each axis maps a ghost index to its source: periodic images
 // (src/setup_mpi.c:62-253: high ghosts <- first active, low ghosts <- last
 // active; a face-normal field keeps its upper boundary face, which CT
-// computes) or the symmetric mirror (src/boundary_conditions.c).  Applying
+// computes), the symmetric mirror (src/boundary_conditions.c) or, on a
+// theta side at a pole, the mirror across the pole rotated by pi in phi.  Applying
 // the three axis maps at once equals the reference's sequential i, j, k
 // passes, because each later pass copies whole rows of the earlier ones, and
 // every source point is interior in all three axes, so no thread reads a
@@ -1683,14 +1688,28 @@ struct GhostArgs {
 __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o) {
   if (x < s) {
     if (klo == GK_BC_PERIODIC) return x + (e - s + 1);
-    if (klo == GK_BC_SYMMETRIC) return 2 * s - 1 - x + o;
+    if (klo == GK_BC_SYMMETRIC || klo == GK_BC_POLE) return 2 * s - 1 - x + o;
     return x;
   }
   if (x > e + o && x <= e + 4 + o) {
     if (khi == GK_BC_PERIODIC) return x - (e - s + 1);
-    if (khi == GK_BC_SYMMETRIC) return 2 * e + 1 + o - x;
+    if (khi == GK_BC_SYMMETRIC || khi == GK_BC_POLE) return 2 * e + 1 + o - x;
   }
   return x;
+}
+
+// Pole boundary (src/spherical/spherical_boundary.c:23-138): a ghost row j
+// beyond a pole takes the row mirrored across it (the symmetric map above) at
+// phi + pi, i.e. k -> s + (k - s + nk/2) mod nk for every k of the box, with
+// the theta and phi components negated.  True when row j of a field with
+// face offset o along j is such a ghost row.
+__device__ __forceinline__ bool pole_row(const Grid &g, int j, int klo, int khi, int o) {
+  return (j < g.s[1] && klo == GK_BC_POLE) ||
+         (j > g.e[1] + o && j <= g.e[1] + 4 + o && khi == GK_BC_POLE);
+}
+__device__ __forceinline__ int pole_k(const Grid &g, int k) {
+  const int nk = g.e[2] - g.s[2] + 1;
+  return g.s[2] + (k - g.s[2] + nk / 2) % nk;
 }
 
 // The points of the box outside the active cells, walked as three slabs:
@@ -1746,16 +1765,22 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
   // sources are active points, which this launch never writes, so every
   // value is read before the first store (a load after a store could not be
   // moved ahead of it)
+  // pole ghost rows: rotated source column, v_theta, v_phi (cell fields 2, 3)
+  // and B_theta, B_phi (face fields 1, 2) negated
+  const bool px = pole_row(g, j, a.bc[0][2], a.bc[0][3], 0);
   const long sx = gmap(i, g.s[0], g.e[0], a.bc[0][0], a.bc[0][1], 0) * g.st[0] +
                   gmap(j, g.s[1], g.e[1], a.bc[0][2], a.bc[0][3], 0) * g.st[1] +
-                  gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0);
+                  (px ? pole_k(g, k) : gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0));
   long sb[3];
+  bool pb[3];
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     int m[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
       m[d] = gmap(x[d], g.s[d], g.e[d], a.bc[1][2 * d], a.bc[1][2 * d + 1], d == f ? 1 : 0);
+    pb[f] = pole_row(g, j, a.bc[1][2], a.bc[1][3], f == 1 ? 1 : 0);
+    if (pb[f]) m[2] = pole_k(g, k);
     sb[f] = m[0] * g.st[0] + m[1] * g.st[1] + m[2];
   }
   double xv[6], bv[3];
@@ -1764,17 +1789,26 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
     if (a.X) {
 #pragma unroll
       for (int m = 0; m < 6; m++) xv[m] = a.X[m * g.fs + sx];
+      if (px) {
+        xv[2] = -xv[2];
+        xv[3] = -xv[3];
+      }
     }
     if (a.X32) {
 #pragma unroll
       for (int m = 0; m < 6; m++) xw[m] = a.X32[m * g.fs + sx];
+      if (px) {
+        xw[2] = -xw[2];
+        xw[3] = -xw[3];
+      }
     }
   }
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     if (sb[f] != c) {
-      if (a.B) bv[f] = a.B[f * g.fs + sb[f]];
-      if (a.B32) bw[f] = a.B32[f * g.fs + sb[f]];
+      const bool neg = pb[f] && f != 0;
+      if (a.B) bv[f] = neg ? -a.B[f * g.fs + sb[f]] : a.B[f * g.fs + sb[f]];
+      if (a.B32) bw[f] = neg ? -a.B32[f * g.fs + sb[f]] : a.B32[f * g.fs + sb[f]];
     }
   }
   if (sx != c) {
@@ -1902,6 +1936,303 @@ __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__
   k_mags_body<true>(args[arg], bx, by, bz);
 }
 
+// ---------------------------------------------------------------------------
+// Pole ring averaging (spherical grids whose theta range reaches a pole;
+// src/spherical/spherical_RingAverage.c, helpers src/solver_func.c:2136-2257;
+// called from src/solver_solve.c:258-260, 308-310, 486-491).  FP64 only.
+// The reference's single-rank gather / broadcast copies are identities, so
+// the kernels work on the new state in place.  Every serial sum runs in one
+// lane in the reference's order, so results match the reference bit for bit;
+// the work is a few rows per pole, negligible beside the step.
+constexpr int RING_N = 256;   // azimuthal cells of a ring row, at most
+constexpr int RING_B = 128;   // blocks of a ring row, at most
+
+struct RingArgs {
+  Grid g;
+  Geo geo;
+  double *X;          // new primitives (6 cell fields)
+  double *B;          // new face B (3 face fields)
+  double *M;          // new cell-centred B
+  double *E;          // edge fields: E_i, E_j, E_k (ring scratch after CT)
+  const double *cs;   // cos and sin of the ring angles: cs[k], cs[nk + k]
+  int nk, rows;
+  int pole[2];        // theta = 0 / theta = pi is a pole
+  int nchunk[16];     // blocks of ring row r (0 at the pole)
+};
+
+__device__ __forceinline__ double ring_fsign(double x) {
+  return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0);
+}
+__device__ __forceinline__ double ring_minmod(double x, double y) {
+  return 0.5 * (ring_fsign(x) + ring_fsign(y)) * fmin(fabs(x), fabs(y));
+}
+
+// limited PPM edge values of the middle block (src/solver_func.c:2148-2185)
+__device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, double fp1, double fp2,
+                                         double &vm, double &vp) {
+  const double dvm2 = fm1 - fm2, dvm1 = f - fm1, dvp1 = fp1 - f, dvp2 = fp2 - fp1;
+  double dvc = 0.5 * (dvm2 + dvm1);
+  double SM = 2.0 * ring_minmod(dvm2, dvm1);
+  const double Sm1 = ring_minmod(dvc, SM);
+  dvc = 0.5 * (dvm1 + dvp1);
+  SM = 2.0 * ring_minmod(dvm1, dvp1);
+  const double Sp1 = ring_minmod(dvc, SM);
+  dvc = 0.5 * (dvp2 + dvp1);
+  SM = 2.0 * ring_minmod(dvp2, dvp1);
+  const double Sp2 = ring_minmod(dvc, SM);
+  vp = 0.5 * (f + fp1) - (Sp2 - Sp1) / 6.0;
+  vm = 0.5 * (f + fm1) - (Sp1 - Sm1) / 6.0;
+  double ap = vp - f, am = vm - f;
+  if (ap * am >= 0.0) {
+    ap = 0.0;
+    am = 0.0;
+  } else {
+    if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+    if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+  }
+  vp = f + ap;
+  vm = f + am;
+}
+
+// Ring rows, one wave per (task, pole, row, i).  Tasks 0..4: primitive
+// rho, v1, v2, v3, p of cell row r (HydroRingAverage_base,
+// src/spherical/spherical_RingAverage.c:237-377); task 5: the B_j flux of
+// the face row beyond cell row r, giving E_i; task 6: the B_i flux of cell
+// row r, giving E_j (MagneticRingAverage_base, :380-547).  Velocities and
+// fluxes keep their m = 0, 1 Fourier part (FilterBaseMode) and block-average
+// the rest (RingBlockAvg: block means, periodic PPM edges, the parabola
+// integrated over each cell).  The flux tasks write the change as edge
+// fields, cumulative sums of the flux differences inside each block, zero on
+// block boundaries, over k = ks .. ke+1.
+__global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
+  __shared__ double q[RING_N];
+  __shared__ double blk[RING_B];
+  __shared__ double co[3];
+  const Grid &g = a.g;
+  const int lane = threadIdx.x;
+  const int task = blockIdx.z % 7, p = blockIdx.z / 7;
+  const int r = blockIdx.y;
+  const int i = g.s[0] + blockIdx.x;
+  if (!a.pole[p] || i > g.e[0] + (task == 6 ? 1 : 0)) return;  // uniform per block
+  const int N = a.nk, ks = g.s[2];
+  const int nch = a.nchunk[r], ncell = N / nch;
+  const int jc = p == 0 ? g.s[1] + r : g.e[1] - r;
+  const int j = task == 5 ? (p == 0 ? jc + 1 : jc) : jc;
+  const long row = i * g.st[0] + j * g.st[1] + ks;
+  const bool flux = task >= 5;
+  const bool filt = flux || (task >= 1 && task <= 3);
+  const double *cs = a.cs, *sn = a.cs + N;
+  double *xr = flux ? nullptr : a.X + (long)task * g.fs + row;
+  for (int k = lane; k < N; k += 64) {
+    if (task == 5)
+      q[k] = a.B[g.fs + row + k] * geo3(a.geo, GK_G_FACE_J, i, j, ks + k);
+    else if (task == 6)
+      q[k] = a.B[row + k] * geo3(a.geo, GK_G_FACE_I, i, j, ks + k);
+    else
+      q[k] = xr[k];
+  }
+  __syncthreads();
+  if (filt) {  // FilterBaseMode (src/solver_func.c:2187-2212)
+    if (lane == 0) {
+      const double scl = 2.0 / N;
+      double a0 = 0.0, a1 = 0.0, b1 = 0.0;
+      for (int k = 0; k < N; k++) {
+        a0 += q[k];
+        a1 += q[k] * cs[k];
+        b1 += q[k] * sn[k];
+      }
+      a1 *= scl;
+      b1 *= scl;
+      a0 *= 0.5 * scl;
+      co[0] = a0;
+      co[1] = a1;
+      co[2] = b1;
+    }
+    __syncthreads();
+    for (int k = lane; k < N; k += 64) q[k] -= co[0] + co[1] * cs[k] + co[2] * sn[k];
+    __syncthreads();
+  }
+  for (int b = lane; b < nch; b += 64) {  // block means
+    double sum = 0.0;
+    for (int c = 0; c < ncell; c++) sum += q[b * ncell + c];
+    blk[b] = sum / ncell;
+  }
+  __syncthreads();
+  double *E = a.E + (task == 5 ? 0 : g.fs) + row;
+  for (int b = lane; b < nch; b += 64) {
+    const double e0 = blk[(b + nch - 2) % nch], e1 = blk[(b + nch - 1) % nch], e2 = blk[b];
+    const double e3 = blk[(b + 1) % nch], e4 = blk[(b + 2) % nch];
+    double fL, fR;
+    ring_ppm(e0, e1, e2, e3, e4, fL, fR);
+    const double A = 3.0 * (fL + fR - 2.0 * e2);
+    const double Bc = 2.0 * (3.0 * e2 - fR - 2.0 * fL);
+    const double C = fL;
+    const int k0 = b * ncell;
+    double sum = 0.0;
+    if (flux) E[k0] = 0.0;
+    for (int c = 0; c < ncell; c++) {
+      const double kl = (double)c + 1.0;
+      const double v = (A / 3.0 / ncell / ncell) * (3 * kl * kl - 3 * kl + 1) +
+                       0.5 * Bc * (2 * kl - 1) / ncell + C;
+      const int k = k0 + c;
+      if (!flux) {
+        xr[k] = filt ? v + (co[0] + co[1] * cs[k] + co[2] * sn[k]) : v;
+      } else if (c < ncell - 1) {
+        sum += v - q[k];
+        E[k + 1] = task == 5 ? -sum : sum;
+      }
+    }
+  }
+  if (flux && lane == 0) E[N] = 0.0;
+}
+
+// E and B pole fixes (src/spherical/spherical_RingAverage.c:153-235): before
+// CT, E_i on the pole edges := its mean over the nk + 1 edges of the ring and
+// E_k := 0; after CT, B_j on the pole face := half the difference of the next
+// face row and its antipode in phi.  One lane per i (serial mean).
+__global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
+  const Grid &g = a.g;
+  const int p = blockIdx.y;
+  const int i = g.s[0] + blockIdx.x * 64 + threadIdx.x;
+  if (!a.pole[p] || i > g.e[0] + 1) return;
+  const int jp = p == 0 ? g.s[1] : g.e[1] + 1;
+  const long row = i * g.st[0] + jp * g.st[1];
+  if (i <= g.e[0]) {
+    double *Ei = a.E + row;
+    double m = 0.0;
+    for (int k = g.s[2]; k <= g.e[2] + 1; k++) m = m + Ei[k];
+    m = m / (a.nk + 1);
+    for (int k = g.s[2]; k <= g.e[2] + 1; k++) Ei[k] = m;
+  }
+  double *Ek = a.E + 2 * g.fs + row;
+  for (int k = g.s[2]; k <= g.e[2]; k++) Ek[k] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
+  const Grid &g = a.g;
+  const int p = blockIdx.y;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const int ni = g.e[0] - g.s[0] + 1;
+  if (!a.pole[p] || t >= (long)ni * a.nk) return;
+  const int i = g.s[0] + (int)(t / a.nk), k = g.s[2] + (int)(t % a.nk);
+  const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
+  double *Bj = a.B + g.fs + i * g.st[0];
+  Bj[jp * g.st[1] + k] = 0.5 * (Bj[jn * g.st[1] + k] - Bj[jn * g.st[1] + pole_k(g, k)]);
+}
+
+// ring edge fields at (i, j, k): E_i on the face rows beyond the averaged
+// cell rows, E_j on the averaged cell rows, zero elsewhere
+__device__ __forceinline__ bool ring_face_row(const RingArgs &a, int j) {
+  const Grid &g = a.g;
+  return (a.pole[0] && j >= g.s[1] + 1 && j <= g.s[1] + a.rows) ||
+         (a.pole[1] && j >= g.e[1] - a.rows + 1 && j <= g.e[1]);
+}
+__device__ __forceinline__ bool ring_cell_row(const RingArgs &a, int j) {
+  const Grid &g = a.g;
+  return (a.pole[0] && j >= g.s[1] && j <= g.s[1] + a.rows - 1) ||
+         (a.pole[1] && j >= g.e[1] - a.rows + 1 && j <= g.e[1]);
+}
+__device__ __forceinline__ double ring_ei(const RingArgs &a, int i, int j, int k) {
+  const Grid &g = a.g;
+  if (i > g.e[0] || !ring_face_row(a, j)) return 0.0;
+  return a.E[i * g.st[0] + j * g.st[1] + k];
+}
+__device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k) {
+  const Grid &g = a.g;
+  if (!ring_cell_row(a, j)) return 0.0;
+  return a.E[g.fs + i * g.st[0] + j * g.st[1] + k];
+}
+
+// Curl update of the ring edge fields (E_k is zero), over the rows they
+// reach: cell rows s..s+rows at theta = 0, e-rows..e at theta = pi
+// (src/spherical/spherical_RingAverage.c:584-615; elsewhere the reference's
+// update subtracts 0 / A and leaves B as it is).  The pole face itself is
+// skipped: its area is zero and the next kernel overwrites it.
+__global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
+  const Grid &g = a.g;
+  const int p = blockIdx.y;
+  const int R1 = a.rows + 1, NI1 = g.e[0] - g.s[0] + 2, NK1 = a.nk + 1;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (!a.pole[p] || t >= (long)NI1 * R1 * NK1) return;
+  const int k = g.s[2] + (int)(t % NK1);
+  const long u = t / NK1;
+  const int jj = (int)(u % R1);
+  const int i = g.s[0] + (int)(u / R1);
+  const int j = p == 0 ? g.s[1] + jj : g.e[1] - a.rows + jj;
+  const int jp = p == 0 ? g.s[1] : g.e[1] + 1;
+  const long c = i * g.st[0] + j * g.st[1] + k;
+  const bool ki = k <= g.e[2], ii = i <= g.e[0];
+  // every input before the first store
+  const double ej = ring_ej(a, i, j, k);
+  const double ej_k = ki ? ring_ej(a, i, j, k + 1) : 0.0;
+  const double ej_i = ii ? ring_ej(a, i + 1, j, k) : 0.0;
+  const double ei = ring_ei(a, i, j, k);
+  const double ei_k = ki ? ring_ei(a, i, j, k + 1) : 0.0;
+  const double ei_j = ring_ei(a, i, j + 1, k);
+  const double bi = a.B[c], bj = a.B[g.fs + c], bk = a.B[2 * g.fs + c];
+  if (ki) {  // B_i, faces i = is..ie+1
+    const double cE = 1.0 * ((0.0 - 0.0) - (ej_k - ej));
+    a.B[c] = bi - cE / geo3(a.geo, GK_G_FACE_I, i, j, k);
+  }
+  if (ii && ki && j != jp) {  // B_j
+    const double cE = 1.0 * ((ei_k - ei) - (0.0 - 0.0));
+    a.B[g.fs + c] = bj - cE / geo3(a.geo, GK_G_FACE_J, i, j, k);
+  }
+  if (ii) {  // B_k, faces k = ks..ke+1
+    const double cE = 1.0 * ((ej_i - ej) - (ei_j - ei));
+    a.B[2 * g.fs + c] = bk - cE / geo3(a.geo, GK_G_FACE_K, i, j, k);
+  }
+}
+
+// After the curl update (src/spherical/spherical_RingAverage.c:617-677): the
+// pole face B_j := the next face row; B_k of the pole cell row minus its mean
+// over the nk + 1 faces of the ring.  One lane per i.
+__global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
+  const Grid &g = a.g;
+  const int p = blockIdx.y;
+  const int i = g.s[0] + blockIdx.x * 64 + threadIdx.x;
+  if (!a.pole[p] || i > g.e[0]) return;
+  const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
+  const int jr = p == 0 ? g.s[1] : g.e[1];
+  double *Bj = a.B + g.fs + i * g.st[0];
+  for (int k = g.s[2]; k <= g.e[2]; k++) Bj[jp * g.st[1] + k] = Bj[jn * g.st[1] + k];
+  double *Bk = a.B + 2 * g.fs + i * g.st[0] + jr * g.st[1];
+  double m = 0.0;
+  for (int k = g.s[2]; k <= g.e[2] + 1; k++) m += Bk[k];
+  m = m / (a.nk + 1);
+  for (int k = g.s[2]; k <= g.e[2] + 1; k++) Bk[k] -= m;
+}
+
+// cell-centred B of the active cells whose faces the ring average changed
+// (cell rows s..s+rows and e-rows..e), as the reference's derived-variable
+// pass computes it (src/solver_func.c:310-322)
+__global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
+  const Grid &g = a.g;
+  const int p = blockIdx.y;
+  const int R1 = a.rows + 1, ni = g.e[0] - g.s[0] + 1;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (!a.pole[p] || t >= (long)ni * R1 * a.nk) return;
+  const int k = g.s[2] + (int)(t % a.nk);
+  const long u = t / a.nk;
+  const int jj = (int)(u % R1);
+  const int i = g.s[0] + (int)(u / R1);
+  const int j = p == 0 ? g.s[1] + jj : g.e[1] - a.rows + jj;
+  const long c = i * g.st[0] + j * g.st[1] + k;
+  const int idx[3] = {i, j, k};
+  double b[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) b[m] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
+#pragma unroll
+  for (int m = 0; m < 3; m++) a.M[m * g.fs + c] = b[m];
+}
+
+// clear the reduction slots (the ring average changes the state after the
+// cell update's fused reduction, so the next dt is reduced again)
+__global__ __launch_bounds__(64) void k_slots_reset(Scal *sc) {
+  sc->smin[threadIdx.x] = DBL_INF_BITS;
+  sc->smax[threadIdx.x] = 0ULL;
+}
+
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
 __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src,
                                               double2 *__restrict__ dst, size_t n) {
@@ -1946,6 +2277,13 @@ struct gk_solver {
   int chunk;           // faces per thread in the marching sweeps
   int emf_chunk;       // edges per thread in the marching EMF kernels
   struct gk_batch *batch;  // batch that owns the stepping, if any
+  int ring;            // pole ring averaging on
+  int ring_pole[2];
+  int ring_rows;
+  int ring_nchunk[16];
+  double *ring_tab;    // cos, sin of the ring angles (2 nk), then the j factor
+                       // of the CFL phi width (N[1])
+  const double *dxkj;  // j factor of the CFL phi width
   std::vector<std::string> names;
   std::vector<double> acc_ms;
   std::vector<int> acc_n;
@@ -1992,8 +2330,26 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
       if (!d->cls[a][b] || d->ncls[a][b] < 1) return GK_ERR_ARG;
   }
   for (int a = 0; a < 2; a++)
-    for (int n = 0; n < 6; n++)
-      if (d->bc[a][n] < 0 || d->bc[a][n] > 2) return GK_ERR_ARG;
+    for (int n = 0; n < 6; n++) {
+      if (d->bc[a][n] < 0 || d->bc[a][n] > 3) return GK_ERR_ARG;
+      // the pole boundary: theta sides at a pole of a ring-averaged grid
+      if (d->bc[a][n] == GK_BC_POLE &&
+          (n < 2 || n > 3 || !d->ring || !d->ring_pole[n - 2]))
+        return GK_ERR_ARG;
+    }
+  if (d->ring) {
+    // spherical FP64 grids, periodic in phi, nk <= 256 azimuthal cells; the
+    // averaged rows of the two poles must not meet
+    if (d->coord != 2 || d->precision != 0) return GK_ERR_ARG;
+    if (d->bc[0][4] != GK_BC_PERIODIC || d->bc[1][4] != GK_BC_PERIODIC) return GK_ERR_ARG;
+    if (d->nk < 8 || d->nk > 256 || d->nk % 2 || d->ring_rows < 1 || d->ring_rows > 16)
+      return GK_ERR_ARG;
+    for (int r = 0; r < d->ring_rows; r++)
+      if (d->ring_nchunk[r] < 2 || d->ring_nchunk[r] > 128 || d->nk % d->ring_nchunk[r])
+        return GK_ERR_ARG;
+    const int np = (d->ring_pole[0] ? 1 : 0) + (d->ring_pole[1] ? 1 : 0);
+    if (d->nj < np * (d->ring_rows + 1) + 1) return GK_ERR_ARG;
+  }
   for (int dim = 0; dim < 3; dim++) {  // periodicity is a property of the axis
     bool p0 = d->bc[0][2 * dim] == GK_BC_PERIODIC, p1 = d->bc[0][2 * dim + 1] == GK_BC_PERIODIC;
     bool q0 = d->bc[1][2 * dim] == GK_BC_PERIODIC, q1 = d->bc[1][2 * dim + 1] == GK_BC_PERIODIC;
@@ -2090,6 +2446,42 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     s->geo.cls[a][1] = s->itables + off_c[a][1];
     s->geo.recwf[a] = nullptr;
   }
+  // ring tables: cos and sin of the ring angles, evaluated on the host as the
+  // reference does (src/solver_func.c:2187-2212), and the j factor of the
+  // CFL phi width, widened nk / nchunk times on averaged rows
+  // (src/spherical/spherical_RingAverage.c:118-147; powers of two, so the
+  // product equals the reference's scaled width exactly)
+  std::vector<double> hr;  // kept until the final stream synchronisation
+  s->dxkj = s->geo.g[GK_G_DXK][1];
+  if (d->ring) {
+    s->ring = 1;
+    s->ring_pole[0] = d->ring_pole[0] ? 1 : 0;
+    s->ring_pole[1] = d->ring_pole[1] ? 1 : 0;
+    s->ring_rows = d->ring_rows;
+    memcpy(s->ring_nchunk, d->ring_nchunk, sizeof(s->ring_nchunk));
+    const int nk = d->nk;
+    const double dth = 2.0 * 3.14159265358979323846 / nk;
+    hr.resize(2 * (size_t)nk + s->g.N[1]);
+    for (int k = 0; k < nk; k++) {
+      const double th = dth * (k + 0.5);
+      hr[k] = std::cos(th);
+      hr[nk + k] = std::sin(th);
+    }
+    double *wj = hr.data() + 2 * nk;
+    for (int j = 0; j < s->g.N[1]; j++) wj[j] = d->geom[GK_G_DXK][1][j];
+    for (int p = 0; p < 2; p++) {
+      if (!s->ring_pole[p]) continue;
+      for (int r = 0; r < s->ring_rows; r++) {
+        const int j = p == 0 ? s->g.s[1] + r : s->g.e[1] - r;
+        wj[j] = (nk / s->ring_nchunk[r]) * d->geom[GK_G_DXK][1][j];
+      }
+    }
+    if ((rc = dev_alloc(s, &s->ring_tab, hr.size())) != GK_OK) return fail(rc);
+    if (hipMemcpyAsync(s->ring_tab, hr.data(), hr.size() * sizeof(double), hipMemcpyHostToDevice,
+                       s->stream) != hipSuccess)
+      return fail(GK_ERR_DEVICE);
+    s->dxkj = s->ring_tab + 2 * nk;
+  }
   std::vector<float> hf;  // kept until the final stream synchronisation
   if (s->precision == 1) {  // FP32 copies of the reconstruction tables
     hf.resize(h.size());
@@ -2146,6 +2538,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->tables);
   (void)hipFree(s->tables_f);
+  (void)hipFree(s->ring_tab);
   (void)hipFree(s->stage);
   (void)hipFree(s->itables);
   for (int b = 0; b < 2; b++) {
@@ -2363,6 +2756,7 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   a.X = s->X[s->cur];
   a.M = s->M[s->b_0];
   a.B = s->B[s->b_0];
+  a.dxkj = s->dxkj;
   a.sc = s->sc;
   hipLaunchKernelGGL(k_dt, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
   if (s->ready == 1) {
@@ -2431,11 +2825,15 @@ void resolve_timers(gk_solver *s) {
 // merged per stage into one work-list launch).
 enum StageKernel {
   SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
-  SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_CT, SK_CELL, SK_GHOST, SK_MAGS, SK_COUNT
+  SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_CT, SK_CELL, SK_GHOST, SK_MAGS,
+  // pole ring averaging (single-grid FP64 path only)
+  SK_POLE_E, SK_POLE_B, SK_RING_ROWS, SK_RING_CURL, SK_RING_FIX, SK_RING_MAGS, SK_RING_DT,
+  SK_COUNT
 };
 static const char *const sk_name[SK_COUNT] = {
     "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
-    "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_ct", "k_cell", "k_ghost", "k_mags"};
+    "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_ct", "k_cell", "k_ghost", "k_mags",
+    "k_pole_e", "k_pole_b", "k_ring_rows", "k_ring_curl", "k_ring_fix", "k_ring_mags", "k_ring_dt"};
 
 union StageArgs {
   SweepArgs sw;
@@ -2444,6 +2842,8 @@ union StageArgs {
   CellArgs ce;
   GhostArgs gh;
   MagsArgs mg;
+  RingArgs rg;
+  DtArgs dt;
   Scal *sc;
 };
 
@@ -2457,6 +2857,10 @@ static size_t stage_arg_size(int kid) {
     case SK_CT: return sizeof(CtArgs);
     case SK_CELL: return sizeof(CellArgs);
     case SK_GHOST: return sizeof(GhostArgs);
+    case SK_RING_DT: return sizeof(DtArgs);
+    case SK_POLE_E: case SK_POLE_B: case SK_RING_ROWS: case SK_RING_CURL: case SK_RING_FIX:
+    case SK_RING_MAGS:
+      return sizeof(RingArgs);
     default: return sizeof(MagsArgs);
   }
 }
@@ -2471,6 +2875,10 @@ static const void *stage_arg_ptr(int kid, const StageArgs &u) {
     case SK_CT: return &u.ct;
     case SK_CELL: return &u.ce;
     case SK_GHOST: return &u.gh;
+    case SK_RING_DT: return &u.dt;
+    case SK_POLE_E: case SK_POLE_B: case SK_RING_ROWS: case SK_RING_CURL: case SK_RING_FIX:
+    case SK_RING_MAGS:
+      return &u.rg;
     default: return &u.mg;
   }
 }
@@ -2643,6 +3051,27 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       out.push_back(L);
     }
   }
+  // pole ring averaging: arguments shared by its kernels
+  RingArgs ra;
+  memset(&ra, 0, sizeof ra);
+  if (s->ring) {
+    ra.g = g;
+    ra.geo = s->geo;
+    ra.X = Xn;
+    ra.B = Bn;
+    ra.M = Mn;
+    ra.E = (double *)s->E;
+    ra.cs = s->ring_tab;
+    ra.nk = s->nk;
+    ra.rows = s->ring_rows;
+    ra.pole[0] = s->ring_pole[0];
+    ra.pole[1] = s->ring_pole[1];
+    memcpy(ra.nchunk, s->ring_nchunk, sizeof(ra.nchunk));
+    L.kid = SK_POLE_E;  // E_i ring mean and E_k := 0 on the pole edges
+    L.grid = dim3((s->ni + 1 + 63) / 64, 2, 1);
+    L.u.rg = ra;
+    out.push_back(L);
+  }
   {
     CtArgs a;
     memset(&a, 0, sizeof a);
@@ -2656,6 +3085,12 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     L.kid = SK_CT;
     L.grid = dim3(row_blocks(S[2], Ee[2] + 1, S[1], Ee[1] + 1), 1, s->ni + 1);
     L.u.ct = a;
+    out.push_back(L);
+  }
+  if (s->ring) {
+    L.kid = SK_POLE_B;  // B_j on the pole faces (their area is zero)
+    L.grid = dim3((unsigned)(((long)s->ni * s->nk + 255) / 256), 2, 1);
+    L.u.rg = ra;
     out.push_back(L);
   }
   {
@@ -2681,6 +3116,24 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     L.kid = SK_CELL;
     L.grid = dim3(row_blocks(S[2], Ee[2], S[1], Ee[1]), 1, (s->ni + CELL_CHUNK - 1) / CELL_CHUNK);
     L.u.ce = a;
+    out.push_back(L);
+  }
+  if (s->ring) {
+    // hydro rows and ring edge fields, curl update, pole-row fixes, then the
+    // cell-centred B of the rows whose faces changed
+    const long band = (long)(s->ring_rows + 1);
+    L.kid = SK_RING_ROWS;
+    L.grid = dim3(s->ni + 1, s->ring_rows, 14);
+    L.u.rg = ra;
+    out.push_back(L);
+    L.kid = SK_RING_CURL;
+    L.grid = dim3((unsigned)(((long)(s->ni + 1) * band * (s->nk + 1) + 255) / 256), 2, 1);
+    out.push_back(L);
+    L.kid = SK_RING_FIX;
+    L.grid = dim3((s->ni + 63) / 64, 2, 1);
+    out.push_back(L);
+    L.kid = SK_RING_MAGS;
+    L.grid = dim3((unsigned)(((long)s->ni * band * s->nk + 255) / 256), 2, 1);
     out.push_back(L);
   }
   {
@@ -2713,6 +3166,24 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     L.u.mg = a;
     out.push_back(L);
   }
+  if (s->ring) {
+    // the next CFL dt and |div B| from the averaged state, with the
+    // ring-widened phi width (get_dt_ring)
+    DtArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = g;
+    a.geo = s->geo;
+    a.b0 = s->b0;
+    a.X = Xn;
+    a.M = Mn;
+    a.B = Bn;
+    a.dxkj = s->dxkj;
+    a.sc = s->sc;
+    L.kid = SK_RING_DT;
+    L.grid = box_grid(s->nk, s->nj, s->ni);
+    L.u.dt = a;
+    out.push_back(L);
+  }
 }
 
 template <typename R>
@@ -2739,6 +3210,16 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
     case SK_MAGS: k_mags_shell<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
+    case SK_POLE_E: k_pole_e<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
+    case SK_POLE_B: k_pole_b<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
+    case SK_RING_ROWS: k_ring_rows<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
+    case SK_RING_CURL: k_ring_curl<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
+    case SK_RING_FIX: k_ring_fix<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
+    case SK_RING_MAGS: k_ring_mags<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
+    case SK_RING_DT:
+      k_slots_reset<<<1, NSLOT, 0, st>>>(L.u.dt.sc);
+      k_dt<<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt);
+      break;
   }
 }
 
@@ -3134,6 +3615,7 @@ extern "C" int gk_batch_destroy(gk_batch *b) {
 extern "C" int gk_batch_add(gk_batch *b, gk_solver *s) {
   if (!b || !s) return GK_ERR_ARG;
   if (s->ready != 3 || s->batch) return GK_ERR_STATE;
+  if (s->ring) return GK_ERR_ARG;  // ring-averaged grids step on their own
   HIPCHK(hipStreamSynchronize(s->stream));
   b->mem.push_back(s);
   s->batch = b;
