@@ -137,6 +137,55 @@ def run_gpu(args):
     return el, kt, sc, cbytes / (cms * 1e-3) / 1e9, fp32
 
 
+RING_KERNELS = ("k_pole_e", "k_pole_b", "k_ring_rows", "k_ring_curl", "k_ring_fix",
+                "k_ring_mags", "k_ring_dt")
+
+
+def time_sphere(kind, dims, seed, warmup, steps):
+    """FP64 steps of a spherical problem; wall time and kernel averages."""
+    import gamera_op as go
+    P = go.Problem(kind, seed, *dims)
+    P.device_open(0)
+    P.device_step(warmup)
+    P.enable_timing(True)
+    t0 = time.perf_counter()
+    P.device_step(steps)
+    el = time.perf_counter() - t0
+    kt = P.kernel_times()
+    P.enable_timing(False)
+    sc = P.device_scalars()
+    P.close()
+    if sc["nonfinite"]:
+        raise RuntimeError("non-finite values in the spherical run")
+    return el, kt, sc
+
+
+def run_ring(args):
+    """Full sphere (theta in [0, pi], log r in [1, 10], phi periodic) with
+    pole ring averaging at both poles, FP64, against the same grid of the
+    spherical shell problem (theta in [pi/8, 7pi/8], no poles): the ring
+    kernels' share of the step and the step rate."""
+    sys.path.insert(0, PKG)
+    import gamera_op as go
+    dims = tuple(args.ring_dims)
+    cells = dims[0] * dims[1] * dims[2]
+    log(f"bench: full sphere {dims} with pole ring averaging, {args.ring_steps} steps")
+    el, kt, sc = time_sphere(go.POLE_SPHERE, dims, args.seed, args.warmup, args.ring_steps)
+    log(f"bench: full sphere {el * 1e3 / args.ring_steps:.3f} ms/step")
+    el_s, kt_s, _ = time_sphere(go.DIPOLE_SHELL, dims, args.seed, args.warmup, args.ring_steps)
+    log(f"bench: shell without poles {el_s * 1e3 / args.ring_steps:.3f} ms/step")
+    ring_ms = {k: round(v, 4) for k, v in kt.items() if k in RING_KERNELS}
+    rows = {32: 2, 64: 4, 128: 8, 256: 16}.get(dims[2])
+    return {"grid": list(dims), "rows_per_pole": rows, "steps": args.ring_steps,
+            "ms_per_step": el * 1e3 / args.ring_steps,
+            "cell_updates_per_s": cells * args.ring_steps / el,
+            "ring_kernel_ms": ring_ms,
+            "ring_share_of_kernel_time": sum(ring_ms.values()) / sum(kt.values()),
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+            "shell_no_poles_ms_per_step": el_s * 1e3 / args.ring_steps,
+            "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]}}
+
+
 ENSEMBLE_SIZES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 32)]
 
 
@@ -260,6 +309,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-instances", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=300)
+    ap.add_argument("--no-ring", action="store_true")
+    ap.add_argument("--ring-only", action="store_true",
+                    help="run only the full-sphere ring-average leg (profiling)")
+    ap.add_argument("--ring-dims", type=int, nargs=3, default=[128, 128, 256])
+    ap.add_argument("--ring-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -272,6 +326,9 @@ def main():
               "(multi-GPU sharding is not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
+        return
+    if args.ring_only:
+        print(json.dumps({"pole_ring": run_ring(args)}), flush=True)
         return
     preparer = None
     if not args.no_serve:
@@ -295,6 +352,7 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fp:
             traffic = json.load(fp).get("bytes_per_launch")
+    ring = None if args.no_ring else run_ring(args)
     ens = None if args.no_ensemble else run_ensemble(args)
     srv = None if args.no_serve else run_serving(args, preparer)
     if preparer is not None:
@@ -328,6 +386,7 @@ def main():
         "copy_kernel_GBs": copy_gbs,
         "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
         "fp32_bulk": fp32,
+        "pole_ring": ring,
         "ensemble": ens,
         "serving": srv,
         "cpu_baseline": cpu,

@@ -1402,6 +1402,9 @@ struct CellArgs {
   const double *Bn;        // face B at n+1
   const void *F[3];        // 9 fields of R per direction
   Scal *sc;
+  int skip[4];             // j rows [skip[0], skip[1]], [skip[2], skip[3]] left out of
+                           // the fused dt / div B reduction (pole ring rows,
+                           // reduced again after the ring average); empty if lo > hi
 };
 
 // Cell update (reference src/solver_solve.c:178-255, 314-484; positivity
@@ -1584,17 +1587,18 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     // next step's CFL dt from the new state (magtot = cell B + B0)
     double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0],
                           bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2], dxi, dxj, dxk, CA);
-    if (dcell == dcell) dtc = fmin(dtc, dcell);  // fmin in the reference ignores NaN
+    const bool red = !(j >= a.skip[0] && j <= a.skip[1]) && !(j >= a.skip[2] && j <= a.skip[3]);
+    if (red && dcell == dcell) dtc = fmin(dtc, dcell);  // fmin in the reference ignores NaN
     // div B of the new face field (with the B0 face normals)
     const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
     double db = ((bf[0][1] + f0[0]) * A1[0] - (bf[0][0] + f0[0]) * A0[0]) / V +
                 ((bf[1][1] + f1[1]) * A1[1] - (bf[1][0] + f1[1]) * A0[1]) / V +
                 ((bf[2][1] + f2[2]) * A1[2] - (bf[2][0] + f2[2]) * A0[2]) / V;
     const double adb = fabs(db);
-    if (adb == adb)
-      divb = fmax(divb, adb);
-    else
+    if (adb != adb)
       bad = true;
+    else if (red)
+      divb = fmax(divb, adb);
   }
   block_reduce(dtc, divb, a.sc, bx * 31 + by * 7 + bz);
   if (bad) atomicOr(&a.sc->nanflag, 1u);
@@ -1630,6 +1634,7 @@ struct DtArgs {
   B0c b0;
   const double *X, *M, *B;
   const double *dxkj;  // j factor of the phi width in the CFL dt
+  int j0, j1;          // rows reduced (blockIdx.y covers them from j0)
   Scal *sc;
 };
 
@@ -1637,10 +1642,10 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
   const Grid &g = a.g;
   const Geo &G = a.geo;
   const int k = g.s[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = g.s[1] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = a.j0 + blockIdx.y * blockDim.y + threadIdx.y;
   const int i = g.s[0] + blockIdx.z;
   double dtc = INFINITY, divb = 0.0;
-  if (k <= g.e[2] && j <= g.e[1]) {
+  if (k <= g.e[2] && j <= a.j1) {
     const long c = i * g.st[0] + j * g.st[1] + k;
     double b[3];
 #pragma unroll
@@ -2068,12 +2073,12 @@ __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
     const double Bc = 2.0 * (3.0 * e2 - fR - 2.0 * fL);
     const double C = fL;
     const int k0 = b * ncell;
+    const double A3 = A / 3.0 / ncell / ncell;  // the same for every cell of the block
     double sum = 0.0;
     if (flux) E[k0] = 0.0;
     for (int c = 0; c < ncell; c++) {
       const double kl = (double)c + 1.0;
-      const double v = (A / 3.0 / ncell / ncell) * (3 * kl * kl - 3 * kl + 1) +
-                       0.5 * Bc * (2 * kl - 1) / ncell + C;
+      const double v = A3 * (3 * kl * kl - 3 * kl + 1) + 0.5 * Bc * (2 * kl - 1) / ncell + C;
       const int k = k0 + c;
       if (!flux) {
         xr[k] = filt ? v + (co[0] + co[1] * cs[k] + co[2] * sn[k]) : v;
@@ -2086,26 +2091,39 @@ __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
   if (flux && lane == 0) E[N] = 0.0;
 }
 
+// Mean of the n values of a row, summed in order by one lane of the wave
+// (the reference's serial loop); the row is staged through LDS by the wave.
+__device__ __forceinline__ double ring_mean(const double *v, int n, double *lds) {
+  for (int k = threadIdx.x; k < n; k += 64) lds[k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int k = 0; k < n; k++) m = m + lds[k];
+    lds[n] = m / n;
+  }
+  __syncthreads();
+  return lds[n];
+}
+
 // E and B pole fixes (src/spherical/spherical_RingAverage.c:153-235): before
 // CT, E_i on the pole edges := its mean over the nk + 1 edges of the ring and
 // E_k := 0; after CT, B_j on the pole face := half the difference of the next
-// face row and its antipode in phi.  One lane per i (serial mean).
+// face row and its antipode in phi.  One wave per (pole, i).
 __global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
+  __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
   const int p = blockIdx.y;
-  const int i = g.s[0] + blockIdx.x * 64 + threadIdx.x;
-  if (!a.pole[p] || i > g.e[0] + 1) return;
+  const int i = g.s[0] + blockIdx.x;
+  if (!a.pole[p] || i > g.e[0] + 1) return;  // uniform per block
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1;
   const long row = i * g.st[0] + jp * g.st[1];
   if (i <= g.e[0]) {
-    double *Ei = a.E + row;
-    double m = 0.0;
-    for (int k = g.s[2]; k <= g.e[2] + 1; k++) m = m + Ei[k];
-    m = m / (a.nk + 1);
-    for (int k = g.s[2]; k <= g.e[2] + 1; k++) Ei[k] = m;
+    double *Ei = a.E + row + g.s[2];
+    const double m = ring_mean(Ei, a.nk + 1, lds);
+    for (int k = threadIdx.x; k <= a.nk; k += 64) Ei[k] = m;
   }
-  double *Ek = a.E + 2 * g.fs + row;
-  for (int k = g.s[2]; k <= g.e[2]; k++) Ek[k] = 0.0;
+  double *Ek = a.E + 2 * g.fs + row + g.s[2];
+  for (int k = threadIdx.x; k < a.nk; k += 64) Ek[k] = 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
@@ -2186,21 +2204,20 @@ __global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
 
 // After the curl update (src/spherical/spherical_RingAverage.c:617-677): the
 // pole face B_j := the next face row; B_k of the pole cell row minus its mean
-// over the nk + 1 faces of the ring.  One lane per i.
+// over the nk + 1 faces of the ring.  One wave per (pole, i).
 __global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
+  __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
   const int p = blockIdx.y;
-  const int i = g.s[0] + blockIdx.x * 64 + threadIdx.x;
-  if (!a.pole[p] || i > g.e[0]) return;
+  const int i = g.s[0] + blockIdx.x;
+  if (!a.pole[p] || i > g.e[0]) return;  // uniform per block
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
   const int jr = p == 0 ? g.s[1] : g.e[1];
-  double *Bj = a.B + g.fs + i * g.st[0];
-  for (int k = g.s[2]; k <= g.e[2]; k++) Bj[jp * g.st[1] + k] = Bj[jn * g.st[1] + k];
-  double *Bk = a.B + 2 * g.fs + i * g.st[0] + jr * g.st[1];
-  double m = 0.0;
-  for (int k = g.s[2]; k <= g.e[2] + 1; k++) m += Bk[k];
-  m = m / (a.nk + 1);
-  for (int k = g.s[2]; k <= g.e[2] + 1; k++) Bk[k] -= m;
+  double *Bj = a.B + g.fs + i * g.st[0] + g.s[2];
+  for (int k = threadIdx.x; k < a.nk; k += 64) Bj[jp * g.st[1] + k] = Bj[jn * g.st[1] + k];
+  double *Bk = a.B + 2 * g.fs + i * g.st[0] + jr * g.st[1] + g.s[2];
+  const double m = ring_mean(Bk, a.nk + 1, lds);
+  for (int k = threadIdx.x; k <= a.nk; k += 64) Bk[k] = lds[k] - m;
 }
 
 // cell-centred B of the active cells whose faces the ring average changed
@@ -2224,13 +2241,6 @@ __global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
   for (int m = 0; m < 3; m++) b[m] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
 #pragma unroll
   for (int m = 0; m < 3; m++) a.M[m * g.fs + c] = b[m];
-}
-
-// clear the reduction slots (the ring average changes the state after the
-// cell update's fused reduction, so the next dt is reduced again)
-__global__ __launch_bounds__(64) void k_slots_reset(Scal *sc) {
-  sc->smin[threadIdx.x] = DBL_INF_BITS;
-  sc->smax[threadIdx.x] = 0ULL;
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -2757,6 +2767,8 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   a.M = s->M[s->b_0];
   a.B = s->B[s->b_0];
   a.dxkj = s->dxkj;
+  a.j0 = s->g.s[1];
+  a.j1 = s->g.e[1];
   a.sc = s->sc;
   hipLaunchKernelGGL(k_dt, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
   if (s->ready == 1) {
@@ -3068,7 +3080,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     ra.pole[1] = s->ring_pole[1];
     memcpy(ra.nchunk, s->ring_nchunk, sizeof(ra.nchunk));
     L.kid = SK_POLE_E;  // E_i ring mean and E_k := 0 on the pole edges
-    L.grid = dim3((s->ni + 1 + 63) / 64, 2, 1);
+    L.grid = dim3(s->ni + 1, 2, 1);
     L.u.rg = ra;
     out.push_back(L);
   }
@@ -3113,6 +3125,18 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
     a.sc = s->sc;
+    a.skip[0] = a.skip[2] = 1;  // empty ranges
+    a.skip[1] = a.skip[3] = 0;
+    if (s->ring) {
+      if (s->ring_pole[0]) {
+        a.skip[0] = S[1];
+        a.skip[1] = S[1] + s->ring_rows;
+      }
+      if (s->ring_pole[1]) {
+        a.skip[2] = Ee[1] - s->ring_rows;
+        a.skip[3] = Ee[1];
+      }
+    }
     L.kid = SK_CELL;
     L.grid = dim3(row_blocks(S[2], Ee[2], S[1], Ee[1]), 1, (s->ni + CELL_CHUNK - 1) / CELL_CHUNK);
     L.u.ce = a;
@@ -3130,7 +3154,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     L.grid = dim3((unsigned)(((long)(s->ni + 1) * band * (s->nk + 1) + 255) / 256), 2, 1);
     out.push_back(L);
     L.kid = SK_RING_FIX;
-    L.grid = dim3((s->ni + 63) / 64, 2, 1);
+    L.grid = dim3(s->ni, 2, 1);
     out.push_back(L);
     L.kid = SK_RING_MAGS;
     L.grid = dim3((unsigned)(((long)s->ni * band * s->nk + 255) / 256), 2, 1);
@@ -3167,8 +3191,9 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     out.push_back(L);
   }
   if (s->ring) {
-    // the next CFL dt and |div B| from the averaged state, with the
-    // ring-widened phi width (get_dt_ring)
+    // the next CFL dt and |div B| of the rows the ring average changed (the
+    // cell update's fused reduction left them out), with the ring-widened phi
+    // width (get_dt_ring); launched once per pole band
     DtArgs a;
     memset(&a, 0, sizeof a);
     a.g = g;
@@ -3180,9 +3205,14 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.dxkj = s->dxkj;
     a.sc = s->sc;
     L.kid = SK_RING_DT;
-    L.grid = box_grid(s->nk, s->nj, s->ni);
-    L.u.dt = a;
-    out.push_back(L);
+    L.grid = box_grid(s->nk, s->ring_rows + 1, s->ni);
+    for (int p = 0; p < 2; p++) {
+      if (!s->ring_pole[p]) continue;
+      a.j0 = p == 0 ? S[1] : Ee[1] - s->ring_rows;
+      a.j1 = a.j0 + s->ring_rows;
+      L.u.dt = a;
+      out.push_back(L);
+    }
   }
 }
 
@@ -3216,10 +3246,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_RING_CURL: k_ring_curl<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
     case SK_RING_FIX: k_ring_fix<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
     case SK_RING_MAGS: k_ring_mags<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
-    case SK_RING_DT:
-      k_slots_reset<<<1, NSLOT, 0, st>>>(L.u.dt.sc);
-      k_dt<<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt);
-      break;
+    case SK_RING_DT: k_dt<<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt); break;
   }
 }
 
