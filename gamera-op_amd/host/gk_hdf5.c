@@ -154,6 +154,12 @@ int dump_extra_data(const char *base) {
   const hsize_t dgeo[4] = {NF_geo, NI, NJ, NK}, drec[4] = {NF_rec, NI, NJ, NK};
   int rc = 0;
   rc |= put(f, "/geo", H5T_NATIVE_DOUBLE, 4, dgeo, &geo[0][0][0][0]);
+  if (doRingAverage == 1) {
+    /* the ring copy of the face areas (src/hdf5_io.c:258-265): on one rank
+       the ring arrays span the whole box, so they are geo's face fields */
+    const hsize_t dring[4] = {3, NI, NJ, NK};
+    rc |= put(f, "/geo_ring", H5T_NATIVE_DOUBLE, 4, dring, &geo[face_idir][0][0][0]);
+  }
   rc |= put(f, "/rec", H5T_NATIVE_DOUBLE, 4, drec, &rec[0][0][0][0]);
   rc |= put3(f, "/B0_i", gem[B0AfaceI_b1]);
   rc |= put3(f, "/B0_j", gem[B0AfaceJ_b2]);

@@ -31,8 +31,12 @@ needs = pytest.mark.skipif(
 
 
 @needs
-def test_restart_reads_reference_dump_and_rewrites_it(tmp_path, monkeypatch):
-    kind, seed, n, steps = 0, 3, (16, 16, 8), 5
+@pytest.mark.parametrize("kind,seed,n,steps", [(0, 3, (16, 16, 8), 5),
+                                               (4, 12, (12, 16, 32), 4)])
+def test_restart_reads_reference_dump_and_rewrites_it(tmp_path, monkeypatch, kind, seed, n,
+                                                      steps):
+    """Cartesian, and a full sphere with pole ring averaging (whose dumps
+    also carry the ring face areas, /geo_ring)."""
     rec = run_reference(kind, seed, *n, steps, workdir=str(tmp_path))
     files = sorted(f for f in os.listdir(tmp_path) if f.endswith(".h5"))
     assert files == ["mhd_00-00-00_000000.h5", "mhd_00-00-00_000001.h5"], files
@@ -62,6 +66,13 @@ def test_restart_reads_reference_dump_and_rewrites_it(tmp_path, monkeypatch):
                        "/x1max_global", "/x2min_global", "/x2max_global", "/x3min_global",
                        "/x3max_global"]:
                 r = subprocess.run([H5DIFF, "mhd_00-00-00_000001.h5", ours, ds, ds],
+                                   capture_output=True, text=True)
+                assert r.returncode == 0, (ds, r.stdout[-2000:])
+            # geometry datasets of the first dump (dump_extra_data)
+            assert h5.dump_extra_data(b"ours") == 0
+            extra = ["/geo", "/rec"] + (["/geo_ring"] if kind == 4 else [])
+            for ds in extra:
+                r = subprocess.run([H5DIFF, "mhd_00-00-00_000000.h5", ours, ds, ds],
                                    capture_output=True, text=True)
                 assert r.returncode == 0, (ds, r.stdout[-2000:])
     finally:
