@@ -2748,6 +2748,12 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
     launch_ghost(s, s->X[1 - s->cur], s->B[s->b_m1]);
     launch_mags(s, s->B[s->b_m1], s->M[s->b_m1]);
   }
+  // Ghost faces that no step writes (a side without a boundary condition)
+  // keep their initial values at every time level, as in the reference's
+  // single in-place array; the X buffers already hold them, and the third
+  // face-B buffer starts as a copy of the current one.
+  HIPCHK(hipMemcpyAsync(s->B[s->b_n], s->B[s->b_0], 3 * fs * sizeof(double),
+                        hipMemcpyDeviceToDevice, s->stream));
   refresh_copies32(s);
   Scal cur;
   HIPCHK(hipMemcpyAsync(&cur, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));

@@ -20,7 +20,7 @@ LIB_OP = os.path.join(HERE, "libgamera_op.so")
 LIB_HOST = os.path.join(HERE, "libgamera_host.so")
 
 # seeded problem generators (gamera-op_amd/problems/gk_seeded.c)
-SINUSOID, SINUSOID_BLAST, DIPOLE_SHELL, SINUSOID_SPHERE, POLE_SPHERE = 0, 1, 2, 3, 4
+SINUSOID, SINUSOID_BLAST, DIPOLE_SHELL, SINUSOID_SPHERE, POLE_SPHERE, POLE_CAP = 0, 1, 2, 3, 4, 5
 
 NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
 COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr",
@@ -188,7 +188,7 @@ class Problem:
     def __init__(self, kind, seed, ni, nj, nk, coord=None, full_b0=False):
         self.h, self.op = _load()
         if coord is None:
-            coord = 2 if kind in (DIPOLE_SHELL, POLE_SPHERE) else 0
+            coord = 2 if kind in (DIPOLE_SHELL, POLE_SPHERE, POLE_CAP) else 0
         rc = self.h.gkh_setup(kind, seed, ni, nj, nk, coord, 1 if full_b0 else 0)
         if rc != 0:
             raise GkError(f"gkh_setup failed ({rc})")

@@ -659,8 +659,9 @@ int gkh_main(const char *config_file, int nt) {
   A1 = A2 = A3 = Field_Default;
   set_problem_config();
   /* coordinate system of the seeded problem: spherical for the dipole
-     shell (kind 2) and the full sphere (kind 4), Cartesian for the others */
-  gk_coord_type = (gk_problem_kind == 2 || gk_problem_kind == 4) ? 2 : 0;
+     shell (kind 2), the full sphere (kind 4) and the polar cap (kind 5),
+     Cartesian for the others */
+  gk_coord_type = (gk_problem_kind == 2 || gk_problem_kind == 4 || gk_problem_kind == 5) ? 2 : 0;
   if (initialize_config(0, 1, config_file) != 0) return -1;
   if (setup_comm_and_types(0) != 0) return -1;
   if (initialize_logging(BASE_LOG_NAME, 0) != 0) return -1;

@@ -32,6 +32,11 @@
  *                      crosses the poles, so every pole ring carries azimuthal
  *                      structure), p = 0.1 plus an over-pressure blob (ratio 50)
  *                      centred near the north pole.
+ *   GK_POLE_CAP        GK_POLE_SPHERE's state on the northern hemisphere,
+ *                      theta in [0, pi/2], with the boundary layout of
+ *                      src/problem_FieldLoop_spherical.c: the pole boundary at
+ *                      theta = 0 only, every other ghost fixed at its initial
+ *                      value (the state is set on the whole box).
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -56,7 +61,9 @@ enum {
   GK_DIPOLE_SHELL = 2,
   GK_SINUSOID_SPHERE = 3,
   GK_POLE_SPHERE = 4,
+  GK_POLE_CAP = 5,
 };
+
 
 /* Pole boundary routines exist only in spherical builds of the reference
    (src/spherical/spherical_boundary.c); weak references keep this plugin
@@ -69,6 +76,10 @@ void gem_bc_pole_j_high(void) __attribute__((weak));
 int gk_problem_kind = -1;
 uint64_t gk_problem_seed = 0;
 int gk_problem_seed_set = 0;
+
+static int gk_polar(void) {
+  return gk_problem_kind == GK_POLE_SPHERE || gk_problem_kind == GK_POLE_CAP;
+}
 
 /* splitmix64 finaliser; the counter hash chains it over the key words. */
 static uint64_t gk_mix64(uint64_t z) {
@@ -122,10 +133,10 @@ static void gk_draw_parameters(void) {
   p_b0[0] = sz * cos(az);
   p_b0[1] = sz * sin(az);
   p_b0[2] = cz;
-  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_problem_kind == GK_POLE_SPHERE) {
+  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_polar()) {
     p_m = 0.5 + 1.5 * gk_scalar(13);
     p_ctr[0] = 3.0 + 4.0 * gk_scalar(14);               /* r */
-    if (gk_problem_kind == GK_POLE_SPHERE)
+    if (gk_polar())
       p_ctr[1] = PI * (0.05 + 0.1 * gk_scalar(15));     /* theta, near the pole */
     else
       p_ctr[1] = PI * (0.375 + 0.25 * gk_scalar(15));   /* theta */
@@ -169,12 +180,13 @@ void set_problem_config() {
   gk_resolve_selection();
   gk_draw_parameters();
   CA = 10.0;
-  doRingAverage = gk_problem_kind == GK_POLE_SPHERE ? 1 : 0;
-  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_problem_kind == GK_POLE_SPHERE) {
+  doRingAverage = gk_polar() ? 1 : 0;
+  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_polar()) {
     x1min_global = 1.0;
     x1max_global = 10.0;
-    x2min_global = gk_problem_kind == GK_POLE_SPHERE ? 0.0 : PI / 8.0;
-    x2max_global = gk_problem_kind == GK_POLE_SPHERE ? PI : 7.0 * PI / 8.0;
+    x2min_global = gk_polar() ? 0.0 : PI / 8.0;
+    x2max_global = gk_problem_kind == GK_POLE_SPHERE ? PI
+                   : gk_problem_kind == GK_POLE_CAP ? PI / 2.0 : 7.0 * PI / 8.0;
     x3min_global = 0.0;
     x3max_global = 2.0 * PI;
     problem_config.Boundary_i = BC_NON_PERIODIC;
@@ -207,7 +219,7 @@ void set_problem_config() {
  * the geometry arrays.
  */
 int gk_problem_custom_grid(void) {
-  if (gk_problem_kind != GK_DIPOLE_SHELL && gk_problem_kind != GK_POLE_SPHERE) return 0;
+  if (gk_problem_kind != GK_DIPOLE_SHELL && !gk_polar()) return 0;
   for (int i = isg; i <= ieg + 1; i++) {
     int gi = i - is + proc_coords[0] * config.ni;
     double r = pow(10.0, (double)gi / (double)config.ni_global);
@@ -236,6 +248,9 @@ void boundary_conditions() {
     gem_bc_symmetric_i_high();
     gem_bc_pole_j_low();
     gem_bc_pole_j_high();
+  } else if (gk_problem_kind == GK_POLE_CAP && gas_bc_pole_j_low) {
+    gas_bc_pole_j_low();
+    gem_bc_pole_j_low();
   }
 }
 
@@ -298,7 +313,7 @@ static void gk_init_dipole_shell(void) {
          Field_Default, Field_Default);
 }
 
-static void gk_init_pole_sphere(void) {
+static void gk_init_pole_sphere(int whole_box) {
   const double twopi = 2.0 * PI;
   double xc = p_ctr[0] * sin(p_ctr[1]) * cos(p_ctr[2]);
   double yc = p_ctr[0] * sin(p_ctr[1]) * sin(p_ctr[2]);
@@ -308,10 +323,13 @@ static void gk_init_pole_sphere(void) {
   double az = twopi * gk_scalar(9);
   double sz = sqrt(fmax(0.0, 1.0 - cz * cz));
   double u[3] = {0.3 * sz * cos(az), 0.3 * sz * sin(az), 0.3 * cz};
+  const int i0 = whole_box ? isg : is, i1 = whole_box ? ieg : ie;
+  const int j0 = whole_box ? jsg : js, j1 = whole_box ? jeg : je;
+  const int k0 = whole_box ? ksg : ks, k1 = whole_box ? keg : ke;
   for (int s = 0; s < NS1; s++) {
-    for (int i = is; i <= ie; i++) {
-      for (int j = js; j <= je; j++) {
-        for (int k = ks; k <= ke; k++) {
+    for (int i = i0; i <= i1; i++) {
+      for (int j = j0; j <= j1; j++) {
+        for (int k = k0; k <= k1; k++) {
           int64_t gi = i - is + proc_coords[0] * config.ni;
           int64_t gj = j - js + proc_coords[1] * config.nj;
           int64_t gk = k - ks + proc_coords[2] * config.nk;
@@ -353,7 +371,10 @@ void problem_init() {
       gk_init_dipole_shell();
       break;
     case GK_POLE_SPHERE:
-      gk_init_pole_sphere();
+      gk_init_pole_sphere(0);
+      break;
+    case GK_POLE_CAP:
+      gk_init_pole_sphere(1);
       break;
     default:
       log_error("unknown seeded problem kind %d", gk_problem_kind);

@@ -33,6 +33,10 @@ STATE_CASES = [
     # full sphere with pole ring averaging (2 and 4 averaged rows per pole)
     (4, 8, 12, 16, 32, 12),
     (4, 9, 8, 12, 64, 8),
+    # northern hemisphere, pole boundary at theta = 0 only, fixed ghosts
+    # elsewhere (the boundary layout of the reference's field-loop problem)
+    (5, 13, 12, 12, 32, 10),
+    (5, 14, 8, 10, 64, 6),
 ]
 GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
               (4, 8, 12, 16, 32)]

@@ -14,7 +14,7 @@ def oracle_from_problem(P, fast=False):
     ni, nj, nk = P.n
     bc = P.bc_table()
     kinds = []
-    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4) else [False, False, True]
+    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5) else [False, False, True]
     for dim in range(3):
         for side in range(2):
             if periodic[dim]:
@@ -36,6 +36,8 @@ def oracle_from_problem(P, fast=False):
         O.gem[f] = P.gem(f)
     if P.kind == 4:  # full sphere: ring averaging at both poles
         O.set_ring(1, 1, 1)
+    if P.kind == 5:  # northern hemisphere: the pole at theta = 0 only
+        O.set_ring(1, 1, 0)
     return O
 
 

@@ -19,6 +19,9 @@ CASES = [
     (go.POLE_SPHERE, 8, 12, 16, 32, 12),
     (go.POLE_SPHERE, 9, 8, 12, 64, 8),
     (go.POLE_SPHERE, 10, 16, 24, 128, 4),
+    # northern hemisphere: pole at theta = 0 only, fixed ghosts elsewhere
+    (go.POLE_CAP, 13, 12, 12, 32, 10),
+    (go.POLE_CAP, 14, 8, 10, 64, 6),
 ]
 
 

@@ -22,7 +22,9 @@ HASHES = json.load(open(os.path.join(GOLD, "ref_hashes.json")))
 STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16, 20),
                (2, 4, 16, 12, 16, 20), (3, 5, 24, 16, 8, 40),
                # full sphere, pole ring averaging on (2 and 4 rows per pole)
-               (4, 8, 12, 16, 32, 12), (4, 9, 8, 12, 64, 8)]
+               (4, 8, 12, 16, 32, 12), (4, 9, 8, 12, 64, 8),
+               # northern hemisphere: pole at theta = 0 only, fixed ghosts elsewhere
+               (5, 13, 12, 12, 32, 10), (5, 14, 8, 10, 64, 6)]
 
 
 def sha(a):
