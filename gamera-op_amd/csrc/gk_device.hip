@@ -1690,17 +1690,28 @@ struct GhostArgs {
   int bc[2][6];
 };
 
+__device__ __forceinline__ bool mirror_kind(int kind) {
+  return kind == GK_BC_SYMMETRIC || kind == GK_BC_POLE || kind == GK_BC_REFLECTIVE;
+}
 __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o) {
   if (x < s) {
     if (klo == GK_BC_PERIODIC) return x + (e - s + 1);
-    if (klo == GK_BC_SYMMETRIC || klo == GK_BC_POLE) return 2 * s - 1 - x + o;
+    if (mirror_kind(klo)) return 2 * s - 1 - x + o;
+    if (klo == GK_BC_EXTRAPOLATED) return s;
     return x;
   }
   if (x > e + o && x <= e + 4 + o) {
     if (khi == GK_BC_PERIODIC) return x - (e - s + 1);
-    if (khi == GK_BC_SYMMETRIC || khi == GK_BC_POLE) return 2 * e + 1 + o - x;
+    if (mirror_kind(khi)) return 2 * e + 1 + o - x;
+    if (khi == GK_BC_EXTRAPOLATED) return e + o;
   }
   return x;
+}
+
+// true when x is a ghost of a reflective side (cell fields): the velocity
+// component normal to that side changes sign
+__device__ __forceinline__ bool reflect_ghost(int x, int s, int e, int klo, int khi) {
+  return (x < s && klo == GK_BC_REFLECTIVE) || (x > e && x <= e + 4 && khi == GK_BC_REFLECTIVE);
 }
 
 // Pole boundary (src/spherical/spherical_boundary.c:23-138): a ghost row j
@@ -1773,6 +1784,9 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
   // pole ghost rows: rotated source column, v_theta, v_phi (cell fields 2, 3)
   // and B_theta, B_phi (face fields 1, 2) negated
   const bool px = pole_row(g, j, a.bc[0][2], a.bc[0][3], 0);
+  bool rg[3];  // reflective ghost along axis d: normal velocity negated
+#pragma unroll
+  for (int d = 0; d < 3; d++) rg[d] = reflect_ghost(x[d], g.s[d], g.e[d], a.bc[0][2 * d], a.bc[0][2 * d + 1]);
   const long sx = gmap(i, g.s[0], g.e[0], a.bc[0][0], a.bc[0][1], 0) * g.st[0] +
                   gmap(j, g.s[1], g.e[1], a.bc[0][2], a.bc[0][3], 0) * g.st[1] +
                   (px ? pole_k(g, k) : gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0));
@@ -1798,6 +1812,9 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
         xv[2] = -xv[2];
         xv[3] = -xv[3];
       }
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (rg[d]) xv[1 + d] = -xv[1 + d];
     }
     if (a.X32) {
 #pragma unroll
@@ -1806,6 +1823,9 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
         xw[2] = -xw[2];
         xw[3] = -xw[3];
       }
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (rg[d]) xw[1 + d] = -xw[1 + d];
     }
   }
 #pragma unroll
@@ -2341,7 +2361,8 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
   for (int a = 0; a < 2; a++)
     for (int n = 0; n < 6; n++) {
-      if (d->bc[a][n] < 0 || d->bc[a][n] > 3) return GK_ERR_ARG;
+      if (d->bc[a][n] < 0 || d->bc[a][n] > 5) return GK_ERR_ARG;
+      if (a == 1 && d->bc[a][n] == GK_BC_REFLECTIVE) return GK_ERR_ARG;  // gas only
       // the pole boundary: theta sides at a pole of a ring-averaged grid
       if (d->bc[a][n] == GK_BC_POLE &&
           (n < 2 || n > 3 || !d->ring || !d->ring_pole[n - 2]))

@@ -476,12 +476,39 @@ static void mirror(int is_gem, int dim, int side, int negate_normal) {
   }
 }
 
+/* zero-gradient ghosts: every ghost layer copies the first active layer
+   (low side) or the last active layer / boundary face (high side) */
+static void extrapolate(int is_gem, int dim, int side) {
+  int N[3] = {config.NI, config.NJ, config.NK};
+  int n = N[dim];
+  int t1 = dim == 0 ? 1 : 0, t2 = dim == 2 ? 1 : 2;
+  for (int f = 0; f < 6; f++) {
+    double ***a = is_gem ? gem[f] : gas[0][f];
+    int o = is_gem ? (dim == 0 ? gem_onface_i[f] : dim == 1 ? gem_onface_j[f] : gem_onface_k[f])
+                   : (dim == 0 ? gas_onface_i[f] : dim == 1 ? gas_onface_j[f] : gas_onface_k[f]);
+    int src = side == 0 ? NG : n - 1 - NG - 1 + o;
+    for (int g = 0; g < NG; g++) {
+      int dst = side == 0 ? g : n - 1 - NG + g + o;
+      for (int u = 0; u < N[t1]; u++)
+        for (int v = 0; v < N[t2]; v++) {
+          int d3[3], s3[3];
+          d3[dim] = dst;
+          s3[dim] = src;
+          d3[t1] = s3[t1] = u;
+          d3[t2] = s3[t2] = v;
+          a[d3[0]][d3[1]][d3[2]] = a[s3[0]][s3[1]][s3[2]];
+        }
+    }
+  }
+}
+
 static void bc_request(int is_gem, int dim, int side, int kind) {
   if (dim_periodic(dim)) return; /* MPI neighbour exists: nothing to do */
   gk_bc_kind[is_gem][2 * dim + side] = kind;
   if (gk_bc_record_only || !gas || !gem) return;
   if (kind == GK_BCK_SYMMETRIC) mirror(is_gem, dim, side, 0);
   if (kind == GK_BCK_REFLECTIVE) mirror(is_gem, dim, side, 1);
+  if (kind == GK_BCK_EXTRAPOLATED) extrapolate(is_gem, dim, side);
 }
 
 #define GK_BC_DEF(arr, isg_, kind, KIND)                                       \

@@ -6,9 +6,10 @@
  * the facade calls the plugin's boundary_conditions() once with the host
  * arrays, applies each requested routine there, and records the kind per
  * array/face so the device ghost-fill kernel applies the same mirror every
- * step.  Symmetric, reflective (gas only, as in the reference) and none are
- * supported on the device; fixed and extrapolated are accepted on the host and
- * rejected when the device solver is created.
+ * step.  Symmetric, reflective (gas only, as in the reference), extrapolated,
+ * pole (curvilinear.h) and none are supported on the device; fixed is
+ * accepted on the host and rejected when the device solver is created (its
+ * constants leave p_S untouched, so its corners depend on the call order).
  */
 #ifndef COMMON_H
 #define COMMON_H

@@ -37,6 +37,11 @@
  *                      src/problem_FieldLoop_spherical.c: the pole boundary at
  *                      theta = 0 only, every other ghost fixed at its initial
  *                      value (the state is set on the whole box).
+ *   GK_WALL_BOX        GK_SINUSOID_SPHERE in the unit box with reflective
+ *                      walls in x (face B there keeps its initial ghosts: the
+ *                      reference's gem reflective routines are empty),
+ *                      extrapolated (zero-gradient) boundaries in y, periodic
+ *                      in z.
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -62,6 +67,7 @@ enum {
   GK_SINUSOID_SPHERE = 3,
   GK_POLE_SPHERE = 4,
   GK_POLE_CAP = 5,
+  GK_WALL_BOX = 6,
 };
 
 
@@ -199,8 +205,9 @@ void set_problem_config() {
     x2max_global = 1.0;
     x3min_global = 0.0;
     x3max_global = 1.0;
-    problem_config.Boundary_i = BC_PERIODIC;
-    problem_config.Boundary_j = BC_PERIODIC;
+    const int walls = gk_problem_kind == GK_WALL_BOX;
+    problem_config.Boundary_i = walls ? BC_NON_PERIODIC : BC_PERIODIC;
+    problem_config.Boundary_j = walls ? BC_NON_PERIODIC : BC_PERIODIC;
     problem_config.Boundary_k = BC_PERIODIC;
   }
   /* the background field is a property of the problem, not of the initial
@@ -251,6 +258,15 @@ void boundary_conditions() {
   } else if (gk_problem_kind == GK_POLE_CAP && gas_bc_pole_j_low) {
     gas_bc_pole_j_low();
     gem_bc_pole_j_low();
+  } else if (gk_problem_kind == GK_WALL_BOX) {
+    gas_bc_reflective_i_low();
+    gas_bc_reflective_i_high();
+    gas_bc_extrapolated_j_low();
+    gas_bc_extrapolated_j_high();
+    gem_bc_reflective_i_low();
+    gem_bc_reflective_i_high();
+    gem_bc_extrapolated_j_low();
+    gem_bc_extrapolated_j_high();
   }
 }
 
@@ -365,6 +381,7 @@ void problem_init() {
       gk_init_sinusoid(1);
       break;
     case GK_SINUSOID_SPHERE:
+    case GK_WALL_BOX:
       gk_init_sinusoid(1);
       break;
     case GK_DIPOLE_SHELL:

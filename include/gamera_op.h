@@ -39,8 +39,18 @@ typedef enum {
    gem_bc_pole_j_*, src/spherical/spherical_boundary.c:23-138): each ghost row
    takes the row mirrored across the pole, rotated by pi in phi
    (k -> 4 + (k - 4 + nk/2) mod nk), for every i and k of the box, with v_theta,
-   v_phi and B_theta, B_phi negated.  It needs ring averaging (below). */
-enum { GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2, GK_BC_POLE = 3 };
+   v_phi and B_theta, B_phi negated.  It needs ring averaging (below).
+   GK_BC_REFLECTIVE (gas only; gas_bc_reflective_*,
+   src/boundary_conditions.c:114-245): the symmetric mirror with the normal
+   velocity negated.  GK_BC_EXTRAPOLATED (gas_bc_extrapolated_*,
+   gem_bc_extrapolated_*, src/boundary_conditions.c:404-496,687-770): every
+   ghost layer copies the first active layer (low side) or the last active
+   layer or boundary face (high side).  GK_BC_NONE leaves the ghosts at the
+   values uploaded with the state. */
+enum {
+  GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2, GK_BC_POLE = 3,
+  GK_BC_REFLECTIVE = 4, GK_BC_EXTRAPOLATED = 5
+};
 
 /* separable geometry quantities: q(i,j,k) = f[q][0][i] * f[q][1][j] * f[q][2][k] */
 enum {

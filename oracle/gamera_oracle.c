@@ -215,9 +215,9 @@ static void boundary_exchange(go_solver *s) {
 
 /* symmetric mirror for one (dim, side) of fields 0..5
    (src/boundary_conditions.c:9-112 gas, :505-593 gem); every row of the
-   other two axes is mirrored, ghosts included. */
-static void bc_symmetric(go_solver *s, double *base, int dim, int side,
-                         int is_gem) {
+   other two axes is mirrored, ghosts included.  neg: the reflective variant
+   (gas, :114-245), the velocity normal to the side negated. */
+static void bc_mirror(go_solver *s, double *base, int dim, int side, int is_gem, int neg) {
   int N[3] = {s->NI, s->NJ, s->NK};
   int n = N[dim];
   for (int f = 0; f < 6; f++) {
@@ -226,6 +226,17 @@ static void bc_symmetric(go_solver *s, double *base, int dim, int side,
     for (int g = 0; g < NG; g++) {
       int dst = side == 0 ? g : n - 1 - NG + g + o;
       int src = side == 0 ? 2 * NG - 1 - g + o : n - 1 - NG - 1 - g;
+      if (neg && f == gas_v1 + dim) {
+        for (int i = 0; i < s->NI; i++)
+          for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+              int d3[3] = {i, j, k}, s3[3] = {i, j, k};
+              if (d3[dim] != dst) continue;
+              s3[dim] = src;
+              a[IDX(i, j, k)] = -a[IDX(s3[0], s3[1], s3[2])];
+            }
+        continue;
+      }
       if (dim == 0) {
         memcpy(&a[IDX(dst, 0, 0)], &a[IDX(src, 0, 0)],
                sizeof(double) * (size_t)s->NJ * s->NK);
@@ -263,6 +274,31 @@ static void bc_pole(go_solver *s, double *base, int side, int is_gem) {
   }
 }
 
+/* extrapolated (zero-gradient) ghosts for one (dim, side) of fields 0..5
+   (src/boundary_conditions.c:404-496 gas, :687-770 gem): every ghost layer
+   copies the first active layer (low) or the last active layer or boundary
+   face (high), over every row of the other two axes. */
+static void bc_extrapolated(go_solver *s, double *base, int dim, int side, int is_gem) {
+  int N[3] = {s->NI, s->NJ, s->NK};
+  int n = N[dim];
+  for (int f = 0; f < 6; f++) {
+    double *a = &base[(int64_t)f * s->n3];
+    int o = (is_gem && f == dim) ? 1 : 0;
+    int src = side == 0 ? NG : n - 1 - NG - 1 + o;
+    for (int g = 0; g < NG; g++) {
+      int dst = side == 0 ? g : n - 1 - NG + g + o;
+      for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+          for (int k = 0; k < s->NK; k++) {
+            int d3[3] = {i, j, k}, s3[3] = {i, j, k};
+            if (d3[dim] != dst) continue;
+            s3[dim] = src;
+            a[IDX(i, j, k)] = a[IDX(s3[0], s3[1], s3[2])];
+          }
+    }
+  }
+}
+
 /* boundary_conditions(): gas i low/high, j low/high, k low/high, then gem in
    the same order -- the order the shipped plugins call the BC library. */
 static void boundary_conditions(go_solver *s) {
@@ -270,7 +306,11 @@ static void boundary_conditions(go_solver *s) {
     for (int dim = 0; dim < 3; dim++)
       for (int side = 0; side < 2; side++) {
         if (s->bc[2 * dim + side] == GO_BC_SYMMETRIC)
-          bc_symmetric(s, arr ? s->gem : s->gas, dim, side, arr);
+          bc_mirror(s, arr ? s->gem : s->gas, dim, side, arr, 0);
+        if (s->bc[2 * dim + side] == GO_BC_REFLECTIVE && !arr)
+          bc_mirror(s, s->gas, dim, side, 0, 1);
+        if (s->bc[2 * dim + side] == GO_BC_EXTRAPOLATED)
+          bc_extrapolated(s, arr ? s->gem : s->gas, dim, side, arr);
         if (s->bc[2 * dim + side] == GO_BC_POLE && dim == 1)
           bc_pole(s, arr ? s->gem : s->gas, side, arr);
       }

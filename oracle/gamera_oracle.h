@@ -36,8 +36,12 @@ enum {
 };
 
 /* boundary kinds per (dimension, side); GO_BC_POLE only on the j sides of
-   a spherical grid whose theta range ends at a pole */
-enum { GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2, GO_BC_POLE = 3 };
+   a spherical grid whose theta range ends at a pole; GO_BC_REFLECTIVE acts
+   on gas only (the reference's gem reflective routines are empty) */
+enum {
+  GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2, GO_BC_POLE = 3,
+  GO_BC_REFLECTIVE = 4, GO_BC_EXTRAPOLATED = 5
+};
 
 /* coord: 0 Cartesian, 2 spherical (selects the geometric source terms). */
 go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],

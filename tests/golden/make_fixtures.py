@@ -37,6 +37,8 @@ STATE_CASES = [
     # elsewhere (the boundary layout of the reference's field-loop problem)
     (5, 13, 12, 12, 32, 10),
     (5, 14, 8, 10, 64, 6),
+    # reflective walls in x, extrapolated boundaries in y, periodic z
+    (6, 15, 16, 12, 8, 12),
 ]
 GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
               (4, 8, 12, 16, 32)]

@@ -14,13 +14,16 @@ def oracle_from_problem(P, fast=False):
     ni, nj, nk = P.n
     bc = P.bc_table()
     kinds = []
-    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5) else [False, False, True]
+    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5, 6) else [False, False, True]
+    # recorded gas kinds (host) -> oracle kinds: symmetric, reflective,
+    # extrapolated, pole; anything else leaves the ghosts as they are
+    to_oracle = {1: 1, 2: 4, 4: 5, 6: 3}
     for dim in range(3):
         for side in range(2):
             if periodic[dim]:
                 kinds.append(0)
             else:
-                kinds.append({1: 1, 6: 3}.get(bc[0][2 * dim + side], 2))
+                kinds.append(to_oracle.get(bc[0][2 * dim + side], 2))
     O = Oracle(ni, nj, nk, coord=P.coord, bc=tuple(kinds), CA=P.CA, fast=fast)
     for f in range(28):
         O.geo[f] = P.geo(f)

@@ -26,6 +26,9 @@ CASES = [
     # two lanes' rows within a wave
     (go.SINUSOID, 6, 8, 12, 64, 10),
     (go.DIPOLE_SHELL, 7, 12, 8, 70, 10),
+    # reflective walls in x, extrapolated boundaries in y, periodic z
+    (go.WALL_BOX, 15, 16, 12, 8, 12),
+    (go.WALL_BOX, 16, 12, 16, 12, 8),
 ]
 
 

@@ -73,6 +73,11 @@ def test_boundary_kinds_recorded_from_plugin():
         assert P.bc_table().tolist() == [[1, 1, 6, 6, 0, 0], [1, 1, 6, 6, 0, 0]]
     finally:
         P.close()
+    P = go.Problem(go.WALL_BOX, 1, 12, 10, 8)
+    try:  # 2 = reflective (gem: recorded as none, 5), 4 = extrapolated
+        assert P.bc_table().tolist() == [[2, 2, 4, 4, 0, 0], [5, 5, 4, 4, 0, 0]]
+    finally:
+        P.close()
     P = go.Problem(go.SINUSOID, 1, 12, 10, 8)
     try:
         assert P.bc_table().tolist() == [[0] * 6, [0] * 6]

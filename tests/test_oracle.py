@@ -24,7 +24,9 @@ STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16,
                # full sphere, pole ring averaging on (2 and 4 rows per pole)
                (4, 8, 12, 16, 32, 12), (4, 9, 8, 12, 64, 8),
                # northern hemisphere: pole at theta = 0 only, fixed ghosts elsewhere
-               (5, 13, 12, 12, 32, 10), (5, 14, 8, 10, 64, 6)]
+               (5, 13, 12, 12, 32, 10), (5, 14, 8, 10, 64, 6),
+               # reflective walls in x, extrapolated boundaries in y
+               (6, 15, 16, 12, 8, 12)]
 
 
 def sha(a):
