@@ -3811,7 +3811,11 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   const size_t fs = s->fs;
   const size_t nX = 12 * fs, nB = 9 * fs, nM = 9 * fs;
   double *save = nullptr;
-  if (hipMalloc((void **)&save, (nX + nB + nM) * sizeof(double) + sizeof(Scal)) != hipSuccess)
+  // stream-ordered snapshot: allocating and freeing it on the solver's own
+  // stream does not wait for work in flight elsewhere on the device (a batch
+  // quantum the serving loop launched before admitting this grid)
+  if (hipMallocAsync((void **)&save, (nX + nB + nM) * sizeof(double) + sizeof(Scal), s->stream) !=
+      hipSuccess)
     return GK_ERR_NOMEM;
   const Rot r0 = rot_of(s);
   auto snap = [&](bool restore) -> int {
@@ -3851,7 +3855,7 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   auto trial = [&](float &ms) -> int {
     one_step(s);
     ms = 1e30f;
-    for (int rep = 0; rep < repeats; rep++) {
+    for (int rep = 0; rep < repeats; rep++) {  // back to back; one restore after
       float t_ms = 0.0f;
       (void)hipEventRecord(e0, s->stream);
       for (int t = 0; t < trial_steps; t++) one_step(s);
@@ -3859,10 +3863,8 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
       if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t_ms, e0, e1) != hipSuccess)
         return GK_ERR_DEVICE;
       ms = std::min(ms, t_ms);
-      int rc2 = snap(true);
-      if (rc2 != GK_OK) return rc2;
     }
-    return GK_OK;
+    return snap(true);
   };
   // march length of the sweeps, then of the edge-field kernels
   const int cand[] = {8, 16, 32, 64, 128, 256};
@@ -3896,7 +3898,8 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   s->timing = keep_timing;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(save);
+  (void)hipFreeAsync(save, s->stream);
+  (void)hipStreamSynchronize(s->stream);
   if (rc != GK_OK) return rc;
   s->chunk = pick;
   s->emf_chunk = pick_e;
