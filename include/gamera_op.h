@@ -143,8 +143,10 @@ int gk_kernel_times(gk_solver *s, const char **names, double *ms, int *n);
 /* Copy one intermediate field of the last step to the host (reference
    layout): which 0..2 face fluxes of sweep i/j/k (field 0..8: rho, rho v1..3,
    energy, entropy, magnetic stress 1..3), 3..5 face-averaged velocities of
-   sweep i/j/k (field 0..1), 6 edge electric fields (field 0..2), 7 the
-   cell-centred magnetic field of the current state (field 0..2). */
+   sweep i/j/k (field 0..1), 6 edge electric fields (field 0..2; on a
+   ring-averaged grid the rows next to the poles then hold the ring average's
+   edge fields), 7 the cell-centred magnetic field of the current state
+   (field 0..2). */
 int gk_debug_fetch(gk_solver *s, int which, int field, double *host);
 
 /* device bytes held by the solver */

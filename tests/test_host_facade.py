@@ -122,3 +122,19 @@ def test_read_config_yaml_schema():
             f.write("ni_global: 30\nnj_global: 32\nnk_global: 16\nproc_dims_i: 4\n"
                     "proc_dims_j: 1\nproc_dims_k: 1\n")
         assert h.read_config(p.encode()) != 0  # 30 not divisible by 4
+
+
+def test_ring_average_needs_a_supported_resolution():
+    """The reference's ring table covers nk = 32, 64, 128, 256 (it exits
+    otherwise); the facade refuses to describe such a grid for the device."""
+    P = go.Problem(go.POLE_SPHERE, 1, 8, 12, 48)
+    try:
+        with pytest.raises(go.GkError):
+            P.desc_blob()
+    finally:
+        P.close()
+    P = go.Problem(go.POLE_SPHERE, 1, 8, 12, 32)
+    try:
+        assert len(P.desc_blob()) > 0
+    finally:
+        P.close()
