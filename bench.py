@@ -209,13 +209,16 @@ def run_ensemble(args):
     log(f"bench: ensemble of {len(specs)} grids, set-up")
     solvers = []
     cells = 0
+    picks = {}
     for kind, sd, shape in sorted(specs, key=lambda t: (t[0], t[2])):
         P = go.Problem(kind, sd, *shape)
         S = go.Solver(P, 1)
         P.close()
         S.autotune(2)
+        picks[shape] = (S.get_tuning(go.TUNE_CHUNK), S.get_tuning(go.TUNE_EMF_CHUNK))
         solvers.append(S)
         cells += shape[0] * shape[1] * shape[2]
+    log(f"bench: ensemble launch shapes (faces, edges per thread) {picks}")
     steps = args.ens_steps
     B = go.Batch()
     for S in solvers:

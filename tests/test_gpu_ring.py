@@ -15,13 +15,18 @@ import gamera_op as go
 from parity import compare_states, exact_mismatches, run_oracle
 
 CASES = [
-    # kind, seed, ni, nj, nk, steps: 2, 4 and 8 averaged rows per pole
+    # kind, seed, ni, nj, nk, steps: 2, 4, 8 and 16 averaged rows per pole
     (go.POLE_SPHERE, 8, 12, 16, 32, 12),
     (go.POLE_SPHERE, 9, 8, 12, 64, 8),
     (go.POLE_SPHERE, 10, 16, 24, 128, 4),
+    # nk = 256: 16 rows per pole, the outer ones in 128 blocks of 2 cells
+    (go.POLE_SPHERE, 17, 8, 36, 256, 3),
     # northern hemisphere: pole at theta = 0 only, fixed ghosts elsewhere
     (go.POLE_CAP, 13, 12, 12, 32, 10),
     (go.POLE_CAP, 14, 8, 10, 64, 6),
+    # long runs: every limiter and PPM branch of many steps must agree
+    (go.POLE_SPHERE, 18, 12, 16, 32, 200),
+    (go.POLE_CAP, 19, 8, 10, 64, 200),
 ]
 
 
