@@ -270,8 +270,51 @@ def run_serving(args, preparer):
             "throughput_ratio": srv["cell_updates_per_s"] / pl["cell_updates_per_s"]}
 
 
+def run_reference_cpu(args, threads):
+    """The compiled reference (oracle/_ref/ref_driver_cart, the reference's
+    own sources built where they lie) on a bounded sample of the same
+    generator: runs of 1 and 1 + K steps, whose difference cancels set-up and
+    output.  None when the binary is absent or does not run."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver_cart")
+    if not os.path.exists(exe):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from refrun import YAML
+    n, k = args.ref_n, args.ref_steps
+    log(f"bench: compiled reference {n}^3, 1 and {1 + k} steps, {threads} threads")
+    wall = {}
+    try:
+        for nt in (1, 1 + k):
+            with tempfile.TemporaryDirectory(prefix="gkrefb") as d:
+                with open(os.path.join(d, "c.yaml"), "w") as f:
+                    f.write(YAML.format(ni=n, nj=n, nk=n, nt=nt))
+                env = dict(os.environ, GK_PROBLEM="1", GK_SEED=str(args.seed),
+                           OMP_NUM_THREADS=str(threads))
+                t0 = time.perf_counter()
+                r = subprocess.run([exe, "c.yaml", os.path.join(d, "out.bin"), "0"], cwd=d, env=env,
+                                   capture_output=True, text=True, timeout=600)
+                wall[nt] = time.perf_counter() - t0
+                if r.returncode != 0:
+                    log(f"bench: compiled reference failed ({r.returncode}); port only")
+                    return None
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"bench: compiled reference did not run ({e}); port only")
+        return None
+    per_step = (wall[1 + k] - wall[1]) / k
+    log(f"bench: compiled reference {per_step:.3f} s/step")
+    return {"value": n ** 3 / per_step, "unit": "cell-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{n}^3 cells of the same generator, the compiled reference (its own "
+                      f"sources, gcc -O2 -fopenmp) run for 1 and {1 + k} AB2 steps; value from "
+                      f"the difference of the two wall times"}
+
+
 def run_cpu(args):
-    """CPU restatement on a bounded sample of the same workload."""
+    """CPU baselines on bounded samples of the same workload: the compiled
+    reference when it is present (the reported baseline), and the CPU
+    restatement (port), reported beside it."""
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, PKG)
@@ -290,10 +333,15 @@ def run_cpu(args):
     O.step(steps)
     el = time.perf_counter() - t0
     log(f"bench: CPU {el:.2f} s")
-    return {"value": n ** 3 * steps / el, "unit": "cell-updates/s", "cores": threads,
+    port = {"value": n ** 3 * steps / el, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{n}^3 cells of the same generator, {steps} AB2 steps after 1 warm-up, "
                       f"oracle/liboracle_fast.so (-O3 -march=native, OpenMP)"}
+    ref = None if args.no_ref_cpu else run_reference_cpu(args, threads)
+    if ref is None:
+        return port
+    ref["port"] = port
+    return ref
 
 
 def main():
@@ -306,8 +354,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk comparison run")
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ref-n", type=int, default=64, help="compiled-reference CPU sample size")
+    ap.add_argument("--ref-steps", type=int, default=5)
+    ap.add_argument("--no-ref-cpu", action="store_true",
+                    help="time only the CPU restatement, not the compiled reference")
     ap.add_argument("--chunk", type=int, default=0, help="faces per thread in the i/j sweeps")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-instances", type=int, default=64)
