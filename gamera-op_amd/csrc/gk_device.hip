@@ -1964,9 +1964,11 @@ __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__
 // ---------------------------------------------------------------------------
 // Pole ring averaging (spherical grids whose theta range reaches a pole;
 // src/spherical/spherical_RingAverage.c, helpers src/solver_func.c:2136-2257;
-// called from src/solver_solve.c:258-260, 308-310, 486-491).  FP64 only.
-// The reference's single-rank gather / broadcast copies are identities, so
-// the kernels work on the new state in place.  Every serial sum runs in one
+// called from src/solver_solve.c:258-260, 308-310, 486-491).  The kernels
+// work on the FP64 new state in place (the reference's single-rank gather /
+// broadcast copies are identities); in the FP32-bulk form they also refresh
+// the FP32 copies of what they change, and the edge fields they read or
+// write are the form's (type R).  Every serial sum runs in one
 // lane in the reference's order, so results match the reference bit for bit;
 // the work is a few rows per pole, negligible beside the step.
 constexpr int RING_N = 256;   // azimuthal cells of a ring row, at most
@@ -1978,7 +1980,8 @@ struct RingArgs {
   double *X;          // new primitives (6 cell fields)
   double *B;          // new face B (3 face fields)
   double *M;          // new cell-centred B
-  double *E;          // edge fields: E_i, E_j, E_k (ring scratch after CT)
+  void *E;            // edge fields of type R: E_i, E_j, E_k (ring scratch after CT)
+  float *X32, *B32, *M32;  // FP32 copies of X, B, M (FP32-bulk form), else null
   const double *cs;   // cos and sin of the ring angles: cs[k], cs[nk + k]
   int nk, rows;
   int pole[2];        // theta = 0 / theta = pi is a pole
@@ -2029,6 +2032,7 @@ __device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, doubl
 // integrated over each cell).  The flux tasks write the change as edge
 // fields, cumulative sums of the flux differences inside each block, zero on
 // block boundaries, over k = ks .. ke+1.
+template <typename R>
 __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
   __shared__ double q[RING_N];
   __shared__ double blk[RING_B];
@@ -2083,7 +2087,8 @@ __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
     blk[b] = sum / ncell;
   }
   __syncthreads();
-  double *E = a.E + (task == 5 ? 0 : g.fs) + row;
+  R *E = (R *)a.E + (task == 5 ? 0 : g.fs) + row;
+  float *xr32 = (flux || !a.X32) ? nullptr : a.X32 + (long)task * g.fs + row;
   for (int b = lane; b < nch; b += 64) {
     const double e0 = blk[(b + nch - 2) % nch], e1 = blk[(b + nch - 1) % nch], e2 = blk[b];
     const double e3 = blk[(b + 1) % nch], e4 = blk[(b + 2) % nch];
@@ -2095,26 +2100,29 @@ __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
     const int k0 = b * ncell;
     const double A3 = A / 3.0 / ncell / ncell;  // the same for every cell of the block
     double sum = 0.0;
-    if (flux) E[k0] = 0.0;
+    if (flux) E[k0] = (R)0.0;
     for (int c = 0; c < ncell; c++) {
       const double kl = (double)c + 1.0;
       const double v = A3 * (3 * kl * kl - 3 * kl + 1) + 0.5 * Bc * (2 * kl - 1) / ncell + C;
       const int k = k0 + c;
       if (!flux) {
-        xr[k] = filt ? v + (co[0] + co[1] * cs[k] + co[2] * sn[k]) : v;
+        const double out = filt ? v + (co[0] + co[1] * cs[k] + co[2] * sn[k]) : v;
+        xr[k] = out;
+        if (xr32) xr32[k] = (float)out;
       } else if (c < ncell - 1) {
         sum += v - q[k];
-        E[k + 1] = task == 5 ? -sum : sum;
+        E[k + 1] = (R)(task == 5 ? -sum : sum);
       }
     }
   }
-  if (flux && lane == 0) E[N] = 0.0;
+  if (flux && lane == 0) E[N] = (R)0.0;
 }
 
 // Mean of the n values of a row, summed in order by one lane of the wave
 // (the reference's serial loop); the row is staged through LDS by the wave.
-__device__ __forceinline__ double ring_mean(const double *v, int n, double *lds) {
-  for (int k = threadIdx.x; k < n; k += 64) lds[k] = v[k];
+template <typename T>
+__device__ __forceinline__ double ring_mean(const T *v, int n, double *lds) {
+  for (int k = threadIdx.x; k < n; k += 64) lds[k] = (double)v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = 0.0;
@@ -2129,6 +2137,7 @@ __device__ __forceinline__ double ring_mean(const double *v, int n, double *lds)
 // CT, E_i on the pole edges := its mean over the nk + 1 edges of the ring and
 // E_k := 0; after CT, B_j on the pole face := half the difference of the next
 // face row and its antipode in phi.  One wave per (pole, i).
+template <typename R>
 __global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
@@ -2138,12 +2147,12 @@ __global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1;
   const long row = i * g.st[0] + jp * g.st[1];
   if (i <= g.e[0]) {
-    double *Ei = a.E + row + g.s[2];
+    R *Ei = (R *)a.E + row + g.s[2];
     const double m = ring_mean(Ei, a.nk + 1, lds);
-    for (int k = threadIdx.x; k <= a.nk; k += 64) Ei[k] = m;
+    for (int k = threadIdx.x; k <= a.nk; k += 64) Ei[k] = (R)m;
   }
-  double *Ek = a.E + 2 * g.fs + row + g.s[2];
-  for (int k = threadIdx.x; k < a.nk; k += 64) Ek[k] = 0.0;
+  R *Ek = (R *)a.E + 2 * g.fs + row + g.s[2];
+  for (int k = threadIdx.x; k < a.nk; k += 64) Ek[k] = (R)0.0;
 }
 
 __global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
@@ -2155,7 +2164,9 @@ __global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
   const int i = g.s[0] + (int)(t / a.nk), k = g.s[2] + (int)(t % a.nk);
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
   double *Bj = a.B + g.fs + i * g.st[0];
-  Bj[jp * g.st[1] + k] = 0.5 * (Bj[jn * g.st[1] + k] - Bj[jn * g.st[1] + pole_k(g, k)]);
+  const double b = 0.5 * (Bj[jn * g.st[1] + k] - Bj[jn * g.st[1] + pole_k(g, k)]);
+  Bj[jp * g.st[1] + k] = b;
+  if (a.B32) a.B32[g.fs + i * g.st[0] + jp * g.st[1] + k] = (float)b;
 }
 
 // ring edge fields at (i, j, k): E_i on the face rows beyond the averaged
@@ -2170,15 +2181,17 @@ __device__ __forceinline__ bool ring_cell_row(const RingArgs &a, int j) {
   return (a.pole[0] && j >= g.s[1] && j <= g.s[1] + a.rows - 1) ||
          (a.pole[1] && j >= g.e[1] - a.rows + 1 && j <= g.e[1]);
 }
+template <typename R>
 __device__ __forceinline__ double ring_ei(const RingArgs &a, int i, int j, int k) {
   const Grid &g = a.g;
   if (i > g.e[0] || !ring_face_row(a, j)) return 0.0;
-  return a.E[i * g.st[0] + j * g.st[1] + k];
+  return (double)((const R *)a.E)[i * g.st[0] + j * g.st[1] + k];
 }
+template <typename R>
 __device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k) {
   const Grid &g = a.g;
   if (!ring_cell_row(a, j)) return 0.0;
-  return a.E[g.fs + i * g.st[0] + j * g.st[1] + k];
+  return (double)((const R *)a.E)[g.fs + i * g.st[0] + j * g.st[1] + k];
 }
 
 // Curl update of the ring edge fields (E_k is zero), over the rows they
@@ -2186,6 +2199,7 @@ __device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k
 // (src/spherical/spherical_RingAverage.c:584-615; elsewhere the reference's
 // update subtracts 0 / A and leaves B as it is).  The pole face itself is
 // skipped: its area is zero and the next kernel overwrites it.
+template <typename R>
 __global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
   const Grid &g = a.g;
   const int p = blockIdx.y;
@@ -2201,24 +2215,30 @@ __global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
   const long c = i * g.st[0] + j * g.st[1] + k;
   const bool ki = k <= g.e[2], ii = i <= g.e[0];
   // every input before the first store
-  const double ej = ring_ej(a, i, j, k);
-  const double ej_k = ki ? ring_ej(a, i, j, k + 1) : 0.0;
-  const double ej_i = ii ? ring_ej(a, i + 1, j, k) : 0.0;
-  const double ei = ring_ei(a, i, j, k);
-  const double ei_k = ki ? ring_ei(a, i, j, k + 1) : 0.0;
-  const double ei_j = ring_ei(a, i, j + 1, k);
+  const double ej = ring_ej<R>(a, i, j, k);
+  const double ej_k = ki ? ring_ej<R>(a, i, j, k + 1) : 0.0;
+  const double ej_i = ii ? ring_ej<R>(a, i + 1, j, k) : 0.0;
+  const double ei = ring_ei<R>(a, i, j, k);
+  const double ei_k = ki ? ring_ei<R>(a, i, j, k + 1) : 0.0;
+  const double ei_j = ring_ei<R>(a, i, j + 1, k);
   const double bi = a.B[c], bj = a.B[g.fs + c], bk = a.B[2 * g.fs + c];
   if (ki) {  // B_i, faces i = is..ie+1
     const double cE = 1.0 * ((0.0 - 0.0) - (ej_k - ej));
-    a.B[c] = bi - cE / geo3(a.geo, GK_G_FACE_I, i, j, k);
+    const double b = bi - cE / geo3(a.geo, GK_G_FACE_I, i, j, k);
+    a.B[c] = b;
+    if (a.B32) a.B32[c] = (float)b;
   }
   if (ii && ki && j != jp) {  // B_j
     const double cE = 1.0 * ((ei_k - ei) - (0.0 - 0.0));
-    a.B[g.fs + c] = bj - cE / geo3(a.geo, GK_G_FACE_J, i, j, k);
+    const double b = bj - cE / geo3(a.geo, GK_G_FACE_J, i, j, k);
+    a.B[g.fs + c] = b;
+    if (a.B32) a.B32[g.fs + c] = (float)b;
   }
   if (ii) {  // B_k, faces k = ks..ke+1
     const double cE = 1.0 * ((ej_i - ej) - (ei_j - ei));
-    a.B[2 * g.fs + c] = bk - cE / geo3(a.geo, GK_G_FACE_K, i, j, k);
+    const double b = bk - cE / geo3(a.geo, GK_G_FACE_K, i, j, k);
+    a.B[2 * g.fs + c] = b;
+    if (a.B32) a.B32[2 * g.fs + c] = (float)b;
   }
 }
 
@@ -2233,11 +2253,21 @@ __global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
   if (!a.pole[p] || i > g.e[0]) return;  // uniform per block
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
   const int jr = p == 0 ? g.s[1] : g.e[1];
-  double *Bj = a.B + g.fs + i * g.st[0] + g.s[2];
-  for (int k = threadIdx.x; k < a.nk; k += 64) Bj[jp * g.st[1] + k] = Bj[jn * g.st[1] + k];
-  double *Bk = a.B + 2 * g.fs + i * g.st[0] + jr * g.st[1] + g.s[2];
+  const long oj = g.fs + i * g.st[0] + g.s[2];
+  const long ok = 2 * g.fs + i * g.st[0] + jr * g.st[1] + g.s[2];
+  double *Bj = a.B + oj;
+  for (int k = threadIdx.x; k < a.nk; k += 64) {
+    const double b = Bj[jn * g.st[1] + k];
+    Bj[jp * g.st[1] + k] = b;
+    if (a.B32) a.B32[oj + jp * g.st[1] + k] = (float)b;
+  }
+  double *Bk = a.B + ok;
   const double m = ring_mean(Bk, a.nk + 1, lds);
-  for (int k = threadIdx.x; k <= a.nk; k += 64) Bk[k] = lds[k] - m;
+  for (int k = threadIdx.x; k <= a.nk; k += 64) {
+    const double b = lds[k] - m;
+    Bk[k] = b;
+    if (a.B32) a.B32[ok + k] = (float)b;
+  }
 }
 
 // cell-centred B of the active cells whose faces the ring average changed
@@ -2260,7 +2290,10 @@ __global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
 #pragma unroll
   for (int m = 0; m < 3; m++) b[m] = cell_b(a.B + m * g.fs, a.geo, c, g.st[m], idx[m], m);
 #pragma unroll
-  for (int m = 0; m < 3; m++) a.M[m * g.fs + c] = b[m];
+  for (int m = 0; m < 3; m++) {
+    a.M[m * g.fs + c] = b[m];  // FP64 too in the FP32-bulk form: the band dt reads it
+    if (a.M32) a.M32[m * g.fs + c] = (float)b[m];
+  }
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -2369,9 +2402,9 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
         return GK_ERR_ARG;
     }
   if (d->ring) {
-    // spherical FP64 grids, periodic in phi, nk <= 256 azimuthal cells; the
+    // spherical grids, periodic in phi, nk <= 256 azimuthal cells; the
     // averaged rows of the two poles must not meet
-    if (d->coord != 2 || d->precision != 0) return GK_ERR_ARG;
+    if (d->coord != 2) return GK_ERR_ARG;
     if (d->bc[0][4] != GK_BC_PERIODIC || d->bc[1][4] != GK_BC_PERIODIC) return GK_ERR_ARG;
     if (d->nk < 8 || d->nk > 256 || d->nk % 2 || d->ring_rows < 1 || d->ring_rows > 16)
       return GK_ERR_ARG;
@@ -3099,7 +3132,10 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     ra.X = Xn;
     ra.B = Bn;
     ra.M = Mn;
-    ra.E = (double *)s->E;
+    ra.E = s->E;
+    ra.X32 = f32 ? s->X32[1 - r.cur] : nullptr;
+    ra.B32 = f32 ? s->B32[r.b_n] : nullptr;
+    ra.M32 = f32 ? s->M32[r.b_n] : nullptr;
     ra.cs = s->ring_tab;
     ra.nk = s->nk;
     ra.rows = s->ring_rows;
@@ -3267,10 +3303,10 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
     case SK_MAGS: k_mags_shell<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
-    case SK_POLE_E: k_pole_e<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
+    case SK_POLE_E: k_pole_e<R><<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
     case SK_POLE_B: k_pole_b<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
-    case SK_RING_ROWS: k_ring_rows<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
-    case SK_RING_CURL: k_ring_curl<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
+    case SK_RING_ROWS: k_ring_rows<R><<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
+    case SK_RING_CURL: k_ring_curl<R><<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
     case SK_RING_FIX: k_ring_fix<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
     case SK_RING_MAGS: k_ring_mags<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
     case SK_RING_DT: k_dt<<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt); break;

@@ -94,7 +94,7 @@ typedef struct {
   double b0_lorentz[3];    /* dmagsf_G, constant */
   /* pole ring averaging (doRingAverage; src/spherical/spherical_RingAverage.c,
      called from src/solver_solve.c:258-260,308-310,486-491): ring = 1 turns it
-     on (spherical, FP64, k periodic).  ring_pole[0|1]: theta = 0 / theta = pi
+     on (spherical, k periodic; either precision form).  ring_pole[0|1]: theta = 0 / theta = pi
      is a pole of the grid; the ring_rows cell rows next to each pole are
      averaged, row r (0 at the pole) over ring_nchunk[r] azimuthal blocks, and
      the CFL time step uses the phi width times nk / ring_nchunk[r] on them.

@@ -27,6 +27,8 @@ CASES = [
     (go.SINUSOID_SPHERE, 5, 24, 16, 8, 40),
     # 65 k faces: a 64-face tile of the staged k sweep plus a leftover column
     (go.SINUSOID_BLAST, 6, 16, 12, 64, 30),
+    # reflective walls in x, extrapolated boundaries in y
+    (go.WALL_BOX, 15, 24, 24, 16, 40),
 ]
 
 
