@@ -10,10 +10,13 @@ when the timed region starts.
 
 Prints ONE JSON line with the measured cell-updates/s, the roofline of the
 dominant kernel (HIP events on the solver's stream over the timed region) and
-a CPU baseline (the CPU restatement of the same step, timed on this host's
-cores on a bounded sample).
+a CPU baseline timed on this host's cores on bounded samples: the compiled
+reference when oracle/_ref holds it, with the CPU restatement beside it.
+Further legs: the FP32-bulk form, a full sphere with pole ring averaging, a
+batched ensemble and a served request stream (each can be switched off).
 
-  python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-fp32]
+                  [--no-ring | --ring-only] [--no-ensemble] [--no-serve]
 """
 import argparse
 import json
