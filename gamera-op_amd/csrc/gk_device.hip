@@ -1708,6 +1708,13 @@ __device__ __forceinline__ int gmap(int x, int s, int e, int klo, int khi, int o
   return x;
 }
 
+// true when x is a ghost of a fixed side for a field with face offset o:
+// the ghost holds constants (gas rho = 1, v = 0, p = 1; face B = 0) and p_S
+// follows the maps of the other axes (the fixed axis maps x to itself)
+__device__ __forceinline__ bool fixed_ghost(int x, int s, int e, int klo, int khi, int o) {
+  return (x < s && klo == GK_BC_FIXED) || (x > e + o && x <= e + 4 + o && khi == GK_BC_FIXED);
+}
+
 // true when x is a ghost of a reflective side (cell fields): the velocity
 // component normal to that side changes sign
 __device__ __forceinline__ bool reflect_ghost(int x, int s, int e, int klo, int khi) {
@@ -1785,19 +1792,27 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
   // and B_theta, B_phi (face fields 1, 2) negated
   const bool px = pole_row(g, j, a.bc[0][2], a.bc[0][3], 0);
   bool rg[3];  // reflective ghost along axis d: normal velocity negated
+  bool fx = false;  // in a fixed ghost slab: rho, v, p constant
 #pragma unroll
-  for (int d = 0; d < 3; d++) rg[d] = reflect_ghost(x[d], g.s[d], g.e[d], a.bc[0][2 * d], a.bc[0][2 * d + 1]);
+  for (int d = 0; d < 3; d++) {
+    rg[d] = reflect_ghost(x[d], g.s[d], g.e[d], a.bc[0][2 * d], a.bc[0][2 * d + 1]);
+    fx = fx || fixed_ghost(x[d], g.s[d], g.e[d], a.bc[0][2 * d], a.bc[0][2 * d + 1], 0);
+  }
   const long sx = gmap(i, g.s[0], g.e[0], a.bc[0][0], a.bc[0][1], 0) * g.st[0] +
                   gmap(j, g.s[1], g.e[1], a.bc[0][2], a.bc[0][3], 0) * g.st[1] +
                   (px ? pole_k(g, k) : gmap(k, g.s[2], g.e[2], a.bc[0][4], a.bc[0][5], 0));
   long sb[3];
-  bool pb[3];
+  bool pb[3], fb[3];
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     int m[3];
+    fb[f] = false;  // in a fixed ghost slab of this face field: 0
 #pragma unroll
-    for (int d = 0; d < 3; d++)
+    for (int d = 0; d < 3; d++) {
       m[d] = gmap(x[d], g.s[d], g.e[d], a.bc[1][2 * d], a.bc[1][2 * d + 1], d == f ? 1 : 0);
+      fb[f] = fb[f] || fixed_ghost(x[d], g.s[d], g.e[d], a.bc[1][2 * d], a.bc[1][2 * d + 1],
+                                   d == f ? 1 : 0);
+    }
     pb[f] = pole_row(g, j, a.bc[1][2], a.bc[1][3], f == 1 ? 1 : 0);
     if (pb[f]) m[2] = pole_k(g, k);
     sb[f] = m[0] * g.st[0] + m[1] * g.st[1] + m[2];
@@ -1836,19 +1851,34 @@ __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, c
       if (a.B32) bw[f] = neg ? -a.B32[f * g.fs + sb[f]] : a.B32[f * g.fs + sb[f]];
     }
   }
-  if (sx != c) {
+  if (fx) {  // fixed: the constants replace rho, v, p; p_S as mapped
+    const double fixv[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      xv[m] = fixv[m];
+      xw[m] = (float)fixv[m];
+    }
+  }
+  if (sx != c || fx) {
+    const int nm = sx != c ? 6 : 5;  // p_S mapped to itself is left alone
     if (a.X) {
 #pragma unroll
-      for (int m = 0; m < 6; m++) a.X[m * g.fs + c] = xv[m];
+      for (int m = 0; m < 6; m++)
+        if (m < nm) a.X[m * g.fs + c] = xv[m];
     }
     if (a.X32) {
 #pragma unroll
-      for (int m = 0; m < 6; m++) a.X32[m * g.fs + c] = xw[m];
+      for (int m = 0; m < 6; m++)
+        if (m < nm) a.X32[m * g.fs + c] = xw[m];
     }
   }
 #pragma unroll
   for (int f = 0; f < 3; f++) {
-    if (sb[f] != c) {
+    if (fb[f]) {
+      bv[f] = 0.0;
+      bw[f] = 0.0f;
+    }
+    if (sb[f] != c || fb[f]) {
       if (a.B) a.B[f * g.fs + c] = bv[f];
       if (a.B32) a.B32[f * g.fs + c] = bw[f];
     }
@@ -2394,7 +2424,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
   for (int a = 0; a < 2; a++)
     for (int n = 0; n < 6; n++) {
-      if (d->bc[a][n] < 0 || d->bc[a][n] > 5) return GK_ERR_ARG;
+      if (d->bc[a][n] < 0 || d->bc[a][n] > 6) return GK_ERR_ARG;
       if (a == 1 && d->bc[a][n] == GK_BC_REFLECTIVE) return GK_ERR_ARG;  // gas only
       // the pole boundary: theta sides at a pole of a ring-averaged grid
       if (d->bc[a][n] == GK_BC_POLE &&

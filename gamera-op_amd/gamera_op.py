@@ -21,7 +21,7 @@ LIB_HOST = os.path.join(HERE, "libgamera_host.so")
 
 # seeded problem generators (gamera-op_amd/problems/gk_seeded.c)
 SINUSOID, SINUSOID_BLAST, DIPOLE_SHELL, SINUSOID_SPHERE, POLE_SPHERE, POLE_CAP = 0, 1, 2, 3, 4, 5
-WALL_BOX = 6
+WALL_BOX, INFLOW_BOX = 6, 7
 
 NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
 COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr",

@@ -502,6 +502,29 @@ static void extrapolate(int is_gem, int dim, int side) {
   }
 }
 
+/* fixed ghosts: gas rho = 1, v = 0, p = 1 (p_S untouched), gem fields 0 */
+static void fixed_fill(int is_gem, int dim, int side) {
+  int N[3] = {config.NI, config.NJ, config.NK};
+  int n = N[dim];
+  int t1 = dim == 0 ? 1 : 0, t2 = dim == 2 ? 1 : 2;
+  for (int f = 0; f < (is_gem ? 6 : 5); f++) {
+    double ***a = is_gem ? gem[f] : gas[0][f];
+    int o = is_gem ? (dim == 0 ? gem_onface_i[f] : dim == 1 ? gem_onface_j[f] : gem_onface_k[f]) : 0;
+    double v = is_gem ? 0.0 : (f == gas_rho || f == gas_p ? 1.0 : 0.0);
+    for (int g = 0; g < NG; g++) {
+      int dst = side == 0 ? g : n - 1 - NG + g + o;
+      for (int u = 0; u < N[t1]; u++)
+        for (int w = 0; w < N[t2]; w++) {
+          int d3[3];
+          d3[dim] = dst;
+          d3[t1] = u;
+          d3[t2] = w;
+          a[d3[0]][d3[1]][d3[2]] = v;
+        }
+    }
+  }
+}
+
 static void bc_request(int is_gem, int dim, int side, int kind) {
   if (dim_periodic(dim)) return; /* MPI neighbour exists: nothing to do */
   gk_bc_kind[is_gem][2 * dim + side] = kind;
@@ -509,6 +532,7 @@ static void bc_request(int is_gem, int dim, int side, int kind) {
   if (kind == GK_BCK_SYMMETRIC) mirror(is_gem, dim, side, 0);
   if (kind == GK_BCK_REFLECTIVE) mirror(is_gem, dim, side, 1);
   if (kind == GK_BCK_EXTRAPOLATED) extrapolate(is_gem, dim, side);
+  if (kind == GK_BCK_FIXED) fixed_fill(is_gem, dim, side);
 }
 
 #define GK_BC_DEF(arr, isg_, kind, KIND)                                       \

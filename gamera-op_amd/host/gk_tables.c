@@ -210,6 +210,7 @@ static int dev_bc(int dim, int side, int arr) {
     case GK_BCK_POLE: return dim == 1 ? GK_BC_POLE : -1;
     case GK_BCK_REFLECTIVE: return arr == 0 ? GK_BC_REFLECTIVE : -1;
     case GK_BCK_EXTRAPOLATED: return GK_BC_EXTRAPOLATED;
+    case GK_BCK_FIXED: return GK_BC_FIXED;
     case GK_BCK_UNSET:
     case GK_BCK_NONE: return GK_BC_NONE;
     default: return -1;

@@ -6,10 +6,11 @@
  * the facade calls the plugin's boundary_conditions() once with the host
  * arrays, applies each requested routine there, and records the kind per
  * array/face so the device ghost-fill kernel applies the same mirror every
- * step.  Symmetric, reflective (gas only, as in the reference), extrapolated,
- * pole (curvilinear.h) and none are supported on the device; fixed is
- * accepted on the host and rejected when the device solver is created (its
- * constants leave p_S untouched, so its corners depend on the call order).
+ * step.  Every routine is supported on the device: symmetric, reflective (gas
+ * only, as in the reference), extrapolated, fixed, pole (curvilinear.h) and
+ * none.  Fixed leaves p_S untouched, so where a fixed side meets another one
+ * p_S depends on the call order; the device reproduces the order i, j, k
+ * (low, high) for gas, then gem.
  */
 #ifndef COMMON_H
 #define COMMON_H

@@ -42,6 +42,10 @@
  *                      reference's gem reflective routines are empty),
  *                      extrapolated (zero-gradient) boundaries in y, periodic
  *                      in z.
+ *   GK_INFLOW_BOX      GK_SINUSOID_SPHERE in the unit box with a fixed
+ *                      boundary at x = 0 (rho = 1, v = 0, p = 1, face B = 0
+ *                      in the ghosts), extrapolated at x = 1, symmetric in y,
+ *                      periodic in z.
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -68,6 +72,7 @@ enum {
   GK_POLE_SPHERE = 4,
   GK_POLE_CAP = 5,
   GK_WALL_BOX = 6,
+  GK_INFLOW_BOX = 7,
 };
 
 
@@ -205,7 +210,7 @@ void set_problem_config() {
     x2max_global = 1.0;
     x3min_global = 0.0;
     x3max_global = 1.0;
-    const int walls = gk_problem_kind == GK_WALL_BOX;
+    const int walls = gk_problem_kind == GK_WALL_BOX || gk_problem_kind == GK_INFLOW_BOX;
     problem_config.Boundary_i = walls ? BC_NON_PERIODIC : BC_PERIODIC;
     problem_config.Boundary_j = walls ? BC_NON_PERIODIC : BC_PERIODIC;
     problem_config.Boundary_k = BC_PERIODIC;
@@ -258,6 +263,15 @@ void boundary_conditions() {
   } else if (gk_problem_kind == GK_POLE_CAP && gas_bc_pole_j_low) {
     gas_bc_pole_j_low();
     gem_bc_pole_j_low();
+  } else if (gk_problem_kind == GK_INFLOW_BOX) {
+    gas_bc_fixed_i_low();
+    gas_bc_extrapolated_i_high();
+    gas_bc_symmetric_j_low();
+    gas_bc_symmetric_j_high();
+    gem_bc_fixed_i_low();
+    gem_bc_extrapolated_i_high();
+    gem_bc_symmetric_j_low();
+    gem_bc_symmetric_j_high();
   } else if (gk_problem_kind == GK_WALL_BOX) {
     gas_bc_reflective_i_low();
     gas_bc_reflective_i_high();
@@ -382,6 +396,7 @@ void problem_init() {
       break;
     case GK_SINUSOID_SPHERE:
     case GK_WALL_BOX:
+    case GK_INFLOW_BOX:
       gk_init_sinusoid(1);
       break;
     case GK_DIPOLE_SHELL:

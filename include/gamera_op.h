@@ -45,11 +45,15 @@ typedef enum {
    velocity negated.  GK_BC_EXTRAPOLATED (gas_bc_extrapolated_*,
    gem_bc_extrapolated_*, src/boundary_conditions.c:404-496,687-770): every
    ghost layer copies the first active layer (low side) or the last active
-   layer or boundary face (high side).  GK_BC_NONE leaves the ghosts at the
-   values uploaded with the state. */
+   layer or boundary face (high side).  GK_BC_FIXED (gas_bc_fixed_*,
+   gem_bc_fixed_*, src/boundary_conditions.c:247-403,602-686): the ghost
+   layers hold rho = 1, v = 0, p = 1 and face B = 0; p_S is left to the other
+   sides' maps, as when the plugin calls the library i, j, k (low, high) for
+   gas, then gem.  GK_BC_NONE leaves the ghosts at the values uploaded with
+   the state. */
 enum {
   GK_BC_PERIODIC = 0, GK_BC_SYMMETRIC = 1, GK_BC_NONE = 2, GK_BC_POLE = 3,
-  GK_BC_REFLECTIVE = 4, GK_BC_EXTRAPOLATED = 5
+  GK_BC_REFLECTIVE = 4, GK_BC_EXTRAPOLATED = 5, GK_BC_FIXED = 6
 };
 
 /* separable geometry quantities: q(i,j,k) = f[q][0][i] * f[q][1][j] * f[q][2][k] */

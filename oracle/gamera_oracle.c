@@ -299,6 +299,28 @@ static void bc_extrapolated(go_solver *s, double *base, int dim, int side, int i
   }
 }
 
+/* fixed ghosts for one (dim, side) (src/boundary_conditions.c:247-403 gas,
+   :602-686 gem): gas rho = 1, v1..v3 = 0, p = 1 with p_S left as it is; gem
+   fields 0..5 = 0; over every row of the other two axes. */
+static void bc_fixed(go_solver *s, double *base, int dim, int side, int is_gem) {
+  int N[3] = {s->NI, s->NJ, s->NK};
+  int n = N[dim];
+  for (int f = 0; f < (is_gem ? 6 : 5); f++) {
+    double *a = &base[(int64_t)f * s->n3];
+    int o = (is_gem && f == dim) ? 1 : 0;
+    double v = is_gem ? 0.0 : ((f == gas_rho || f == gas_p) ? 1.0 : 0.0);
+    for (int g = 0; g < NG; g++) {
+      int dst = side == 0 ? g : n - 1 - NG + g + o;
+      for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+          for (int k = 0; k < s->NK; k++) {
+            int d3[3] = {i, j, k};
+            if (d3[dim] == dst) a[IDX(i, j, k)] = v;
+          }
+    }
+  }
+}
+
 /* boundary_conditions(): gas i low/high, j low/high, k low/high, then gem in
    the same order -- the order the shipped plugins call the BC library. */
 static void boundary_conditions(go_solver *s) {
@@ -311,6 +333,8 @@ static void boundary_conditions(go_solver *s) {
           bc_mirror(s, s->gas, dim, side, 0, 1);
         if (s->bc[2 * dim + side] == GO_BC_EXTRAPOLATED)
           bc_extrapolated(s, arr ? s->gem : s->gas, dim, side, arr);
+        if (s->bc[2 * dim + side] == GO_BC_FIXED)
+          bc_fixed(s, arr ? s->gem : s->gas, dim, side, arr);
         if (s->bc[2 * dim + side] == GO_BC_POLE && dim == 1)
           bc_pole(s, arr ? s->gem : s->gas, side, arr);
       }

@@ -40,7 +40,7 @@ enum {
    on gas only (the reference's gem reflective routines are empty) */
 enum {
   GO_BC_PERIODIC = 0, GO_BC_SYMMETRIC = 1, GO_BC_NONE = 2, GO_BC_POLE = 3,
-  GO_BC_REFLECTIVE = 4, GO_BC_EXTRAPOLATED = 5
+  GO_BC_REFLECTIVE = 4, GO_BC_EXTRAPOLATED = 5, GO_BC_FIXED = 6
 };
 
 /* coord: 0 Cartesian, 2 spherical (selects the geometric source terms). */

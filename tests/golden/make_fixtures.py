@@ -39,6 +39,8 @@ STATE_CASES = [
     (5, 14, 8, 10, 64, 6),
     # reflective walls in x, extrapolated boundaries in y, periodic z
     (6, 15, 16, 12, 8, 12),
+    # fixed boundary at x = 0, extrapolated at x = 1, symmetric in y
+    (7, 22, 16, 12, 8, 12),
 ]
 GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
               (4, 8, 12, 16, 32)]

@@ -14,10 +14,10 @@ def oracle_from_problem(P, fast=False):
     ni, nj, nk = P.n
     bc = P.bc_table()
     kinds = []
-    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5, 6) else [False, False, True]
+    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5, 6, 7) else [False, False, True]
     # recorded gas kinds (host) -> oracle kinds: symmetric, reflective,
-    # extrapolated, pole; anything else leaves the ghosts as they are
-    to_oracle = {1: 1, 2: 4, 4: 5, 6: 3}
+    # fixed, extrapolated, pole; anything else leaves the ghosts as they are
+    to_oracle = {1: 1, 2: 4, 3: 6, 4: 5, 6: 3}
     for dim in range(3):
         for side in range(2):
             if periodic[dim]:

@@ -29,6 +29,7 @@ CASES = [
     (go.SINUSOID_BLAST, 6, 16, 12, 64, 30),
     # reflective walls in x, extrapolated boundaries in y
     (go.WALL_BOX, 15, 24, 24, 16, 40),
+    (go.INFLOW_BOX, 24, 24, 16, 16, 30),
 ]
 
 

@@ -29,6 +29,9 @@ CASES = [
     # reflective walls in x, extrapolated boundaries in y, periodic z
     (go.WALL_BOX, 15, 16, 12, 8, 12),
     (go.WALL_BOX, 16, 12, 16, 12, 8),
+    # fixed boundary at x = 0, extrapolated at x = 1, symmetric in y
+    (go.INFLOW_BOX, 22, 16, 12, 8, 12),
+    (go.INFLOW_BOX, 23, 12, 16, 12, 8),
 ]
 
 

@@ -78,6 +78,11 @@ def test_boundary_kinds_recorded_from_plugin():
         assert P.bc_table().tolist() == [[2, 2, 4, 4, 0, 0], [5, 5, 4, 4, 0, 0]]
     finally:
         P.close()
+    P = go.Problem(go.INFLOW_BOX, 1, 12, 10, 8)
+    try:  # 3 = fixed, 4 = extrapolated, 1 = symmetric
+        assert P.bc_table().tolist() == [[3, 4, 1, 1, 0, 0], [3, 4, 1, 1, 0, 0]]
+    finally:
+        P.close()
     P = go.Problem(go.SINUSOID, 1, 12, 10, 8)
     try:
         assert P.bc_table().tolist() == [[0] * 6, [0] * 6]

@@ -26,7 +26,9 @@ STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16,
                # northern hemisphere: pole at theta = 0 only, fixed ghosts elsewhere
                (5, 13, 12, 12, 32, 10), (5, 14, 8, 10, 64, 6),
                # reflective walls in x, extrapolated boundaries in y
-               (6, 15, 16, 12, 8, 12)]
+               (6, 15, 16, 12, 8, 12),
+               # fixed boundary at x = 0, extrapolated at x = 1, symmetric y
+               (7, 22, 16, 12, 8, 12)]
 
 
 def sha(a):
