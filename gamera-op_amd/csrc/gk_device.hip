@@ -1638,12 +1638,13 @@ struct DtArgs {
   Scal *sc;
 };
 
-__global__ __launch_bounds__(256) void k_dt(DtArgs a) {
+__device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const int by,
+                                         const int bz) {
   const Grid &g = a.g;
   const Geo &G = a.geo;
-  const int k = g.s[2] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = a.j0 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = g.s[0] + blockIdx.z;
+  const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
+  const int j = a.j0 + by * blockDim.y + threadIdx.y;
+  const int i = g.s[0] + bz;
   double dtc = INFINITY, divb = 0.0;
   if (k <= g.e[2] && j <= a.j1) {
     const long c = i * g.st[0] + j * g.st[1] + k;
@@ -1668,7 +1669,21 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
                  (Bf[2 * g.fs + c] + f2[2]) * geo3(G, GK_G_FACE_K, i, j, k)) / V;
     divb = fabs(db);
   }
-  block_reduce(dtc, divb, a.sc, blockIdx.x * 31 + blockIdx.y * 7 + blockIdx.z);
+  block_reduce(dtc, divb, a.sc, bx * 31 + by * 7 + bz);
+}
+
+__global__ __launch_bounds__(256) void k_dt(DtArgs a) {
+  k_dt_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+__global__ __launch_bounds__(256) void k_dt_batch(const DtArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_dt_body(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
 }
 
 // ---------------------------------------------------------------------------
@@ -2063,15 +2078,16 @@ __device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, doubl
 // fields, cumulative sums of the flux differences inside each block, zero on
 // block boundaries, over k = ks .. ke+1.
 template <typename R>
-__global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
+__device__ __forceinline__ void k_ring_rows_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   __shared__ double q[RING_N];
   __shared__ double blk[RING_B];
   __shared__ double co[3];
   const Grid &g = a.g;
   const int lane = threadIdx.x;
-  const int task = blockIdx.z % 7, p = blockIdx.z / 7;
-  const int r = blockIdx.y;
-  const int i = g.s[0] + blockIdx.x;
+  const int task = bz % 7, p = bz / 7;
+  const int r = by;
+  const int i = g.s[0] + bx;
   if (!a.pole[p] || i > g.e[0] + (task == 6 ? 1 : 0)) return;  // uniform per block
   const int N = a.nk, ks = g.s[2];
   const int nch = a.nchunk[r], ncell = N / nch;
@@ -2148,6 +2164,22 @@ __global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
   if (flux && lane == 0) E[N] = (R)0.0;
 }
 
+template <typename R>
+__global__ __launch_bounds__(64) void k_ring_rows(RingArgs a) {
+  k_ring_rows_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+template <typename R>
+__global__ __launch_bounds__(64) void k_ring_rows_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_ring_rows_body<R>(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
+}
+
 // Mean of the n values of a row, summed in order by one lane of the wave
 // (the reference's serial loop); the row is staged through LDS by the wave.
 template <typename T>
@@ -2168,11 +2200,12 @@ __device__ __forceinline__ double ring_mean(const T *v, int n, double *lds) {
 // E_k := 0; after CT, B_j on the pole face := half the difference of the next
 // face row and its antipode in phi.  One wave per (pole, i).
 template <typename R>
-__global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
+__device__ __forceinline__ void k_pole_e_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
-  const int p = blockIdx.y;
-  const int i = g.s[0] + blockIdx.x;
+  const int p = by;
+  const int i = g.s[0] + bx;
   if (!a.pole[p] || i > g.e[0] + 1) return;  // uniform per block
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1;
   const long row = i * g.st[0] + jp * g.st[1];
@@ -2185,10 +2218,27 @@ __global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
   for (int k = threadIdx.x; k < a.nk; k += 64) Ek[k] = (R)0.0;
 }
 
-__global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
+template <typename R>
+__global__ __launch_bounds__(64) void k_pole_e(RingArgs a) {
+  k_pole_e_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+template <typename R>
+__global__ __launch_bounds__(64) void k_pole_e_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_pole_e_body<R>(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
+}
+
+__device__ __forceinline__ void k_pole_b_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   const Grid &g = a.g;
-  const int p = blockIdx.y;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const int p = by;
+  const long t = (long)bx * 256 + threadIdx.x;
   const int ni = g.e[0] - g.s[0] + 1;
   if (!a.pole[p] || t >= (long)ni * a.nk) return;
   const int i = g.s[0] + (int)(t / a.nk), k = g.s[2] + (int)(t % a.nk);
@@ -2197,6 +2247,20 @@ __global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
   const double b = 0.5 * (Bj[jn * g.st[1] + k] - Bj[jn * g.st[1] + pole_k(g, k)]);
   Bj[jp * g.st[1] + k] = b;
   if (a.B32) a.B32[g.fs + i * g.st[0] + jp * g.st[1] + k] = (float)b;
+}
+
+__global__ __launch_bounds__(256) void k_pole_b(RingArgs a) {
+  k_pole_b_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+__global__ __launch_bounds__(256) void k_pole_b_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_pole_b_body(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
 }
 
 // ring edge fields at (i, j, k): E_i on the face rows beyond the averaged
@@ -2230,11 +2294,12 @@ __device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k
 // update subtracts 0 / A and leaves B as it is).  The pole face itself is
 // skipped: its area is zero and the next kernel overwrites it.
 template <typename R>
-__global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
+__device__ __forceinline__ void k_ring_curl_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   const Grid &g = a.g;
-  const int p = blockIdx.y;
+  const int p = by;
   const int R1 = a.rows + 1, NI1 = g.e[0] - g.s[0] + 2, NK1 = a.nk + 1;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long t = (long)bx * 256 + threadIdx.x;
   if (!a.pole[p] || t >= (long)NI1 * R1 * NK1) return;
   const int k = g.s[2] + (int)(t % NK1);
   const long u = t / NK1;
@@ -2272,14 +2337,31 @@ __global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_ring_curl(RingArgs a) {
+  k_ring_curl_body<R>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+template <typename R>
+__global__ __launch_bounds__(256) void k_ring_curl_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_ring_curl_body<R>(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
+}
+
 // After the curl update (src/spherical/spherical_RingAverage.c:617-677): the
 // pole face B_j := the next face row; B_k of the pole cell row minus its mean
 // over the nk + 1 faces of the ring.  One wave per (pole, i).
-__global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
+__device__ __forceinline__ void k_ring_fix_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
-  const int p = blockIdx.y;
-  const int i = g.s[0] + blockIdx.x;
+  const int p = by;
+  const int i = g.s[0] + bx;
   if (!a.pole[p] || i > g.e[0]) return;  // uniform per block
   const int jp = p == 0 ? g.s[1] : g.e[1] + 1, jn = p == 0 ? g.s[1] + 1 : g.e[1];
   const int jr = p == 0 ? g.s[1] : g.e[1];
@@ -2300,14 +2382,29 @@ __global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
   }
 }
 
+__global__ __launch_bounds__(64) void k_ring_fix(RingArgs a) {
+  k_ring_fix_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+__global__ __launch_bounds__(64) void k_ring_fix_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_ring_fix_body(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
+}
+
 // cell-centred B of the active cells whose faces the ring average changed
 // (cell rows s..s+rows and e-rows..e), as the reference's derived-variable
 // pass computes it (src/solver_func.c:310-322)
-__global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
+__device__ __forceinline__ void k_ring_mags_body(const RingArgs &a, const int bx, const int by,
+                                         const int bz) {
   const Grid &g = a.g;
-  const int p = blockIdx.y;
+  const int p = by;
   const int R1 = a.rows + 1, ni = g.e[0] - g.s[0] + 1;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long t = (long)bx * 256 + threadIdx.x;
   if (!a.pole[p] || t >= (long)ni * R1 * a.nk) return;
   const int k = g.s[2] + (int)(t % a.nk);
   const long u = t / a.nk;
@@ -2324,6 +2421,20 @@ __global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
     a.M[m * g.fs + c] = b[m];  // FP64 too in the FP32-bulk form: the band dt reads it
     if (a.M32) a.M32[m * g.fs + c] = (float)b[m];
   }
+}
+
+__global__ __launch_bounds__(256) void k_ring_mags(RingArgs a) {
+  k_ring_mags_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// one launch for many grids: block w runs work-list entry w
+__global__ __launch_bounds__(256) void k_ring_mags_batch(const RingArgs *__restrict__ args,
+    const WorkItem *__restrict__ work, int nwork) {
+  int w;
+  if (!xcd_item(nwork, w)) return;
+  const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
+  k_ring_mags_body(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+      __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
 }
 
 // plain copy, 16 bytes per lane per iteration (calibration kernel)
@@ -2925,17 +3036,18 @@ void resolve_timers(gk_solver *s) {
 // One AB2 step as a list of launches.  The same plan drives the single-grid
 // path (launched one by one) and the batched path (launches of many grids
 // merged per stage into one work-list launch).
+// Ids follow the order a step runs them in: a batch launches its stages in
+// id order (the k_pole_* and k_ring_* stages exist on pole ring-averaged grids)
 enum StageKernel {
   SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
-  SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_CT, SK_CELL, SK_GHOST, SK_MAGS,
-  // pole ring averaging (single-grid FP64 path only)
-  SK_POLE_E, SK_POLE_B, SK_RING_ROWS, SK_RING_CURL, SK_RING_FIX, SK_RING_MAGS, SK_RING_DT,
-  SK_COUNT
+  SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_POLE_E, SK_CT, SK_POLE_B, SK_CELL, SK_RING_ROWS,
+  SK_RING_CURL, SK_RING_FIX, SK_RING_MAGS, SK_GHOST, SK_MAGS, SK_RING_DT, SK_COUNT
 };
 static const char *const sk_name[SK_COUNT] = {
     "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
-    "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_ct", "k_cell", "k_ghost", "k_mags",
-    "k_pole_e", "k_pole_b", "k_ring_rows", "k_ring_curl", "k_ring_fix", "k_ring_mags", "k_ring_dt"};
+    "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_pole_e", "k_ct", "k_pole_b",
+    "k_cell", "k_ring_rows", "k_ring_curl", "k_ring_fix", "k_ring_mags", "k_ghost", "k_mags",
+    "k_ring_dt"};
 
 union StageArgs {
   SweepArgs sw;
@@ -3363,6 +3475,13 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
     case SK_CELL: k_cell_batch<R><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
     case SK_GHOST: k_ghost_batch<<<nblocks, blk, 0, st>>>((const GhostArgs *)args, work, nwork); break;
     case SK_MAGS: k_mags_batch<<<nblocks, blk, 0, st>>>((const MagsArgs *)args, work, nwork); break;
+    case SK_POLE_E: k_pole_e_batch<R><<<nblocks, dim3(64), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_POLE_B: k_pole_b_batch<<<nblocks, dim3(256), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_RING_ROWS: k_ring_rows_batch<R><<<nblocks, dim3(64), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_RING_CURL: k_ring_curl_batch<R><<<nblocks, dim3(256), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_RING_FIX: k_ring_fix_batch<<<nblocks, dim3(64), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_RING_MAGS: k_ring_mags_batch<<<nblocks, dim3(256), 0, st>>>((const RingArgs *)args, work, nwork); break;
+    case SK_RING_DT: k_dt_batch<<<nblocks, dim3(64, 4), 0, st>>>((const DtArgs *)args, work, nwork); break;
   }
 }
 
@@ -3735,7 +3854,6 @@ extern "C" int gk_batch_destroy(gk_batch *b) {
 extern "C" int gk_batch_add(gk_batch *b, gk_solver *s) {
   if (!b || !s) return GK_ERR_ARG;
   if (s->ready != 3 || s->batch) return GK_ERR_STATE;
-  if (s->ring) return GK_ERR_ARG;  // ring-averaged grids step on their own
   HIPCHK(hipStreamSynchronize(s->stream));
   b->mem.push_back(s);
   s->batch = b;

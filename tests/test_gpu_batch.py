@@ -21,6 +21,9 @@ MIX = [
     # k columns (k faces, E_i edges) in both forms
     (go.SINUSOID_BLAST, 16, 12, 8, 66, 0),
     (go.SINUSOID, 17, 8, 12, 65, 1),
+    # pole ring-averaged grids (full sphere FP64, polar cap FP32-bulk)
+    (go.POLE_SPHERE, 18, 8, 12, 32, 0),
+    (go.POLE_CAP, 19, 8, 10, 64, 1),
 ]
 
 

@@ -118,23 +118,25 @@ def test_ring_fp32_bulk_matches_fp64_oracle(kind, seed, ni, nj, nk, steps):
 
 
 @pytest.mark.gpu
-def test_ring_grid_restrictions():
-    """Ring-averaged grids step on their own (a batch refuses them)."""
-    print("ring restrictions", flush=True)
+def test_ring_grid_refuses_unsupported_descriptor():
+    """A ring-averaged grid whose averaged rows of the two poles would meet
+    (nj < 2 rows + 3) is refused; one that fits runs in both forms."""
+    print("ring descriptor limits", flush=True)
+    P = go.Problem(go.POLE_SPHERE, 11, 8, 6, 32)  # 2 rows per pole: needs nj >= 7
+    try:
+        with pytest.raises(go.GkError):
+            go.Solver(P, 0)
+    finally:
+        P.close()
     P = go.Problem(go.POLE_SPHERE, 11, 8, 12, 32)
     try:
-        S32 = go.Solver(P, 1)
-        S32.close()
-        S = go.Solver(P, 0)
-        B = go.Batch()
-        try:
-            with pytest.raises(go.GkError):
-                B.add(S)
-            S.step(2)
-            prim, _ = S.state()
-            assert np.isfinite(prim[0]).all()
-        finally:
-            B.close()
-            S.close()
+        for prec in (0, 1):
+            S = go.Solver(P, prec)
+            try:
+                S.step(2)
+                prim, _ = S.state()
+                assert np.isfinite(prim[0]).all()
+            finally:
+                S.close()
     finally:
         P.close()
