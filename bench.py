@@ -144,11 +144,11 @@ RING_KERNELS = ("k_pole_e", "k_pole_b", "k_ring_rows", "k_ring_curl", "k_ring_fi
                 "k_ring_mags", "k_ring_dt")
 
 
-def time_sphere(kind, dims, seed, warmup, steps):
-    """FP64 steps of a spherical problem; wall time and kernel averages."""
+def time_sphere(kind, dims, seed, warmup, steps, precision=0):
+    """Steps of a spherical problem; wall time and kernel averages."""
     import gamera_op as go
     P = go.Problem(kind, seed, *dims)
-    P.device_open(0)
+    P.device_open(precision)
     P.device_step(warmup)
     P.enable_timing(True)
     t0 = time.perf_counter()
@@ -167,7 +167,8 @@ def run_ring(args):
     """Full sphere (theta in [0, pi], log r in [1, 10], phi periodic) with
     pole ring averaging at both poles, FP64, against the same grid of the
     spherical shell problem (theta in [pi/8, 7pi/8], no poles): the ring
-    kernels' share of the step and the step rate."""
+    kernels' share of the step and the step rate; and the full sphere in the
+    FP32-bulk form."""
     sys.path.insert(0, PKG)
     import gamera_op as go
     dims = tuple(args.ring_dims)
@@ -177,6 +178,8 @@ def run_ring(args):
     log(f"bench: full sphere {el * 1e3 / args.ring_steps:.3f} ms/step")
     el_s, kt_s, _ = time_sphere(go.DIPOLE_SHELL, dims, args.seed, args.warmup, args.ring_steps)
     log(f"bench: shell without poles {el_s * 1e3 / args.ring_steps:.3f} ms/step")
+    el_32, _, _ = time_sphere(go.POLE_SPHERE, dims, args.seed, args.warmup, args.ring_steps, 1)
+    log(f"bench: full sphere, FP32-bulk {el_32 * 1e3 / args.ring_steps:.3f} ms/step")
     ring_ms = {k: round(v, 4) for k, v in kt.items() if k in RING_KERNELS}
     rows = {32: 2, 64: 4, 128: 8, 256: 16}.get(dims[2])
     return {"grid": list(dims), "rows_per_pole": rows, "steps": args.ring_steps,
@@ -186,6 +189,7 @@ def run_ring(args):
             "ring_share_of_kernel_time": sum(ring_ms.values()) / sum(kt.values()),
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
             "shell_no_poles_ms_per_step": el_s * 1e3 / args.ring_steps,
+            "fp32_bulk_ms_per_step": el_32 * 1e3 / args.ring_steps,
             "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]}}
 
 
