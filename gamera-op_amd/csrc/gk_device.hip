@@ -2202,6 +2202,7 @@ __device__ __forceinline__ double ring_mean(const T *v, int n, double *lds) {
 template <typename R>
 __device__ __forceinline__ void k_pole_e_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  (void)bz;  // 2-D launch
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
   const int p = by;
@@ -2236,6 +2237,7 @@ __global__ __launch_bounds__(64) void k_pole_e_batch(const RingArgs *__restrict_
 
 __device__ __forceinline__ void k_pole_b_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
   const long t = (long)bx * 256 + threadIdx.x;
@@ -2296,6 +2298,7 @@ __device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k
 template <typename R>
 __device__ __forceinline__ void k_ring_curl_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
   const int R1 = a.rows + 1, NI1 = g.e[0] - g.s[0] + 2, NK1 = a.nk + 1;
@@ -2358,6 +2361,7 @@ __global__ __launch_bounds__(256) void k_ring_curl_batch(const RingArgs *__restr
 // over the nk + 1 faces of the ring.  One wave per (pole, i).
 __device__ __forceinline__ void k_ring_fix_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  (void)bz;  // 2-D launch
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
   const int p = by;
@@ -2401,6 +2405,7 @@ __global__ __launch_bounds__(64) void k_ring_fix_batch(const RingArgs *__restric
 // pass computes it (src/solver_func.c:310-322)
 __device__ __forceinline__ void k_ring_mags_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
   const int R1 = a.rows + 1, ni = g.e[0] - g.s[0] + 1;
