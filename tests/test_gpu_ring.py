@@ -140,3 +140,31 @@ def test_ring_grid_refuses_unsupported_descriptor():
                 S.close()
     finally:
         P.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ring_solver_from_descriptor_blob(precision):
+    """A ring-averaged grid handed over as a descriptor blob and its initial
+    state (how the serving loop's worker processes hand over requests) steps
+    bitwise like the solver built from the problem directly."""
+    print(f"ring blob precision={precision}", flush=True)
+    P = go.Problem(go.POLE_SPHERE, 25, 8, 12, 32)
+    try:
+        blob = P.desc_blob()
+        prim = [np.ascontiguousarray(P.gas(f)) for f in range(6)]
+        bface = [np.ascontiguousarray(P.gem(f)) for f in range(3)]
+        A = go.Solver.from_parts(blob, prim, bface, precision)
+        B = go.Solver(P, precision)
+        try:
+            A.step(6)
+            B.step(6)
+            pa, ba = A.state()
+            pb, bb = B.state()
+            assert all(np.array_equal(x, y) for x, y in zip(pa + ba, pb + bb))
+            assert A.scalars()["time"] == B.scalars()["time"]
+        finally:
+            A.close()
+            B.close()
+    finally:
+        P.close()
