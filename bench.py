@@ -12,6 +12,9 @@ Prints ONE JSON line with the measured cell-updates/s, the roofline of the
 dominant kernel (HIP events on the solver's stream over the timed region) and
 a CPU baseline timed on this host's cores on bounded samples: the compiled
 reference when oracle/_ref holds it, with the CPU restatement beside it.
+`host_buffers` gives the rate of the same steps with the device set-up,
+the upload of the host state and the read-back included (the facade's
+solve(nt) path, state in host arrays); it is never `value`.
 Further legs: the FP32-bulk form, a full sphere with pole ring averaging, a
 batched ensemble and a served request stream (each can be switched off).
 
@@ -87,7 +90,9 @@ def run_form(args, precision):
     t0 = time.time()
     P = go.Problem(go.SINUSOID_BLAST, args.seed, n, n, n)
     log(f"bench: host setup {time.time() - t0:.1f} s; device open")
-    P.device_open(precision)
+    t0 = time.perf_counter()
+    P.device_open(precision)  # device tables, state upload from host arrays, prologue
+    t_open = time.perf_counter() - t0
     if args.chunk:
         P.set_tuning(go.TUNE_CHUNK, args.chunk)
     log(f"bench: device holds {P.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
@@ -103,14 +108,16 @@ def run_form(args, precision):
     log(f"bench: {form}: {el * 1e3 / args.steps:.3f} ms/step; scalars {sc}")
     if sc["nonfinite"]:
         raise RuntimeError("non-finite values in the solution")
-    P.device_fetch()
+    t0 = time.perf_counter()
+    P.device_fetch()  # both time levels back into the host arrays
+    t_fetch = time.perf_counter() - t0
     a, b, c = slice(4, 4 + n), slice(4, 4 + n), slice(4, 4 + n)
     state = {"rho": P.gas(0)[a, b, c].copy(), "p": P.gas(4)[a, b, c].copy(),
              "rhov1": (P.gas(0)[a, b, c] * P.gas(1)[a, b, c]),
              "Bi": P.gem(0)[4:5 + n, b, c].copy(), "Bj": P.gem(1)[a, 4:5 + n, c].copy(),
              "Bk": P.gem(2)[a, b, 4:5 + n].copy()}
     P.close()
-    return el, kt, sc, state
+    return el, kt, sc, state, (t_open, t_fetch)
 
 
 def rel_l1(x, y):
@@ -122,10 +129,10 @@ def run_gpu(args):
     sys.path.insert(0, PKG)
     import gamera_op as go
 
-    el, kt, sc, st64 = run_form(args, 0)
+    el, kt, sc, st64, io = run_form(args, 0)
     fp32 = None
     if not args.no_fp32:
-        el32, kt32, sc32, st32 = run_form(args, 1)
+        el32, kt32, sc32, st32, _ = run_form(args, 1)
         cells = args.n ** 3
         fp32 = {"value": cells * args.steps / el32, "unit": "cell-updates/s",
                 "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": el / el32,
@@ -137,7 +144,7 @@ def run_gpu(args):
     log("bench: copy calibration")
     cbytes = 2 * 8 * (args.n + 9) ** 3 * 6  # read + write of six fields
     cms = go.copy_calibrate(cbytes // 2, 20)
-    return el, kt, sc, cbytes / (cms * 1e-3) / 1e9, fp32
+    return el, kt, sc, cbytes / (cms * 1e-3) / 1e9, fp32, io
 
 
 RING_KERNELS = ("k_pole_e", "k_pole_b", "k_ring_rows", "k_ring_curl", "k_ring_fix",
@@ -399,7 +406,7 @@ def main():
         import serve
         preparer = serve.Preparer(args.serve_workers or None)
 
-    el, kt, sc, copy_gbs, fp32 = run_gpu(args)
+    el, kt, sc, copy_gbs, fp32, io = run_gpu(args)
     n = args.n
     cells = n ** 3
     value = cells * args.steps / el
@@ -446,6 +453,12 @@ def main():
                           "kernel_ms_per_step": step_ms_kernels},
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "copy_kernel_GBs": copy_gbs,
+        # the facade's solve(nt) path: state handed over in host arrays
+        "host_buffers": {"value": cells * args.steps / (io[0] + el + io[1]),
+                         "unit": "cell-updates/s", "steps": args.steps,
+                         "open_upload_prologue_s": io[0], "readback_s": io[1],
+                         "note": "timed steps plus device set-up, upload of the host "
+                                 "state, prologue and read-back of both time levels"},
         "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
         "fp32_bulk": fp32,
         "pole_ring": ring,
