@@ -176,7 +176,10 @@ int finalize_solver() {
 /* ------------------------------------------------------------------------ */
 /* host versions of the derived-variable and CFL routines
    (src/solver_func.c:255-383), used on downloaded state */
+/* The host passes below are elementwise or exact min/max reductions, so
+   splitting them over OpenMP threads leaves every result unchanged. */
 void get_derived_variables() {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int i = isg; i <= ieg; i++)
     for (int j = jsg; j <= jeg; j++)
       for (int k = ksg; k <= keg; k++) {
@@ -190,6 +193,7 @@ void get_derived_variables() {
           gem[magtot_b1 + m][i][j][k] = gem[mags_b1 + m][i][j][k] + gem[B0_b1 + m][i][j][k];
       }
   double dmax = 0.0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(max : dmax)
   for (int i = is; i <= ie; i++)
     for (int j = js; j <= je; j++)
       for (int k = ks; k <= ke; k++) {
@@ -497,6 +501,7 @@ int gkh_device_create(int precision, gk_solver **out) {
 /* cell-centred B of the previous time level (what the reference's
    X_p := X copy leaves in mags_p), for dumps */
 static void host_mags_prev(void) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int i = isg; i <= ieg; i++)
     for (int j = jsg; j <= jeg; j++)
       for (int k = ksg; k <= keg; k++) {
