@@ -28,7 +28,8 @@
 // requires the same roundings, not just the same formulas.
 //
 // Memory: every field is one [i][j][k] block with k fastest; the k pitch is
-// padded to a multiple of 16 doubles so rows start 128-byte aligned.  Geometry
+// padded to whole 128-byte lines and every field block is placed so that the
+// first active cell of each row starts a line.  Geometry
 // lives in per-axis tables (orthogonal grids are separable), so per-cell HBM
 // traffic is state only.  The AB2 history is kept by rotating buffers
 // (X^n, X^{n-1}; three face-B and cell-B buffers) instead of copying.  The
@@ -2465,6 +2466,7 @@ struct gk_solver {
   hipStream_t stream;
   double *tables;      // device copy of all per-axis tables
   int *itables;        // class maps
+  std::vector<void *> field_mem;  // allocations behind the field blocks below
   double *X[2];        // 6 fields each: X[cur] = X^n, X[1-cur] = X^{n-1}
   float *X32[2];       // FP32 copies of X, B, M read by the FP32-bulk stencils
   float *B32[3];
@@ -2519,6 +2521,20 @@ static int dev_alloc_bytes(gk_solver *s, void **p, size_t nbytes) {
 
 static int dev_alloc(gk_solver *s, double **p, size_t ndouble) {
   return dev_alloc_bytes(s, (void **)p, ndouble * sizeof(double));
+}
+
+// A block of whole fields.  Its start is placed 128 - NGH * esz bytes into
+// the allocation, so that the first active cell of every row (k = NGH) begins
+// a 128-byte line (the row pitch is a whole number of lines): a wave's 64
+// consecutive cells of a row then span 4 lines in FP64 and 2 in FP32, not
+// 5 and 3.
+static int field_alloc(gk_solver *s, void **p, size_t nbytes, size_t esz) {
+  void *raw = nullptr;
+  const int rc = dev_alloc_bytes(s, &raw, nbytes + 128);
+  if (raw) s->field_mem.push_back(raw);
+  if (rc != GK_OK) return rc;
+  *p = (char *)raw + (128 - NGH * esz % 128) % 128;
+  return GK_OK;
 }
 
 extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
@@ -2582,7 +2598,9 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     s->g.s[a] = NGH;
     s->g.e[a] = NGH + n[a] - 1;
   }
-  const long KP = (s->g.N[2] + 15) / 16 * 16;
+  // row pitch: a whole number of 128-byte lines in the arithmetic type
+  const long KA = d->precision == 1 ? 32 : 16;
+  const long KP = (s->g.N[2] + KA - 1) / KA * KA;
   s->g.st[2] = 1;
   s->g.st[1] = KP;
   s->g.st[0] = KP * s->g.N[1];
@@ -2705,28 +2723,26 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
 
   const size_t fs = (size_t)s->fs;
+  const size_t d8 = sizeof(double), f4 = sizeof(float);
   for (int b = 0; b < 2; b++)
-    if ((rc = dev_alloc(s, &s->X[b], 6 * fs)) != GK_OK) return fail(rc);
+    if ((rc = field_alloc(s, (void **)&s->X[b], 6 * fs * d8, d8)) != GK_OK) return fail(rc);
   for (int b = 0; b < 3; b++) {
-    if ((rc = dev_alloc(s, &s->B[b], 3 * fs)) != GK_OK) return fail(rc);
-    if ((rc = dev_alloc(s, &s->M[b], 3 * fs)) != GK_OK) return fail(rc);
+    if ((rc = field_alloc(s, (void **)&s->B[b], 3 * fs * d8, d8)) != GK_OK) return fail(rc);
+    if ((rc = field_alloc(s, (void **)&s->M[b], 3 * fs * d8, d8)) != GK_OK) return fail(rc);
   }
   if (s->precision == 1) {
     for (int b = 0; b < 2; b++)
-      if ((rc = dev_alloc_bytes(s, (void **)&s->X32[b], 6 * fs * sizeof(float))) != GK_OK)
-        return fail(rc);
+      if ((rc = field_alloc(s, (void **)&s->X32[b], 6 * fs * f4, f4)) != GK_OK) return fail(rc);
     for (int b = 0; b < 3; b++) {
-      if ((rc = dev_alloc_bytes(s, (void **)&s->B32[b], 3 * fs * sizeof(float))) != GK_OK)
-        return fail(rc);
-      if ((rc = dev_alloc_bytes(s, (void **)&s->M32[b], 3 * fs * sizeof(float))) != GK_OK)
-        return fail(rc);
+      if ((rc = field_alloc(s, (void **)&s->B32[b], 3 * fs * f4, f4)) != GK_OK) return fail(rc);
+      if ((rc = field_alloc(s, (void **)&s->M32[b], 3 * fs * f4, f4)) != GK_OK) return fail(rc);
     }
   }
   for (int d2 = 0; d2 < 3; d2++) {
-    if ((rc = dev_alloc_bytes(s, &s->F[d2], 9 * fs * s->esz)) != GK_OK) return fail(rc);
-    if ((rc = dev_alloc_bytes(s, &s->V[d2], 2 * fs * s->esz)) != GK_OK) return fail(rc);
+    if ((rc = field_alloc(s, &s->F[d2], 9 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
+    if ((rc = field_alloc(s, &s->V[d2], 2 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
   }
-  if ((rc = dev_alloc_bytes(s, &s->E, 3 * fs * s->esz)) != GK_OK) return fail(rc);
+  if ((rc = field_alloc(s, &s->E, 3 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
   if (hipMalloc((void **)&s->sc, sizeof(Scal)) != hipSuccess) return fail(GK_ERR_NOMEM);
   Scal z{};
   if (hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream) != hipSuccess)
@@ -2751,21 +2767,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   (void)hipFree(s->ring_tab);
   (void)hipFree(s->stage);
   (void)hipFree(s->itables);
-  for (int b = 0; b < 2; b++) {
-    (void)hipFree(s->X[b]);
-    (void)hipFree(s->X32[b]);
-  }
-  for (int b = 0; b < 3; b++) {
-    (void)hipFree(s->B[b]);
-    (void)hipFree(s->M[b]);
-    (void)hipFree(s->B32[b]);
-    (void)hipFree(s->M32[b]);
-  }
-  for (int d = 0; d < 3; d++) {
-    (void)hipFree(s->F[d]);
-    (void)hipFree(s->V[d]);
-  }
-  (void)hipFree(s->E);
+  for (void *p : s->field_mem) (void)hipFree(p);
   (void)hipFree(s->sc);
   for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
