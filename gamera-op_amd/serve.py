@@ -82,10 +82,14 @@ def device_bytes_estimate(shape, precision):
     """Device bytes gk_create allocates for a grid (state, history,
     intermediates; the per-axis tables are negligible)."""
     N = [n + 9 for n in shape]
-    kp = (N[2] + 15) // 16 * 16
+    ka = 32 if precision == 1 else 16  # row pitch in whole 128-byte lines
+    kp = (N[2] + ka - 1) // ka * ka
     fs = N[0] * N[1] * kp
     esz = 4 if precision == 1 else 8
-    return 8 * fs * (12 + 9 + 9) + esz * fs * (27 + 6 + 3)
+    # FP64 state X (2 x 6), face and cell B (3 x 3 each); intermediates F, V, E;
+    # the FP32-bulk form adds FP32 copies of X, B and cell B
+    copies = 4 * fs * (12 + 9 + 9) if precision == 1 else 0
+    return 8 * fs * (12 + 9 + 9) + esz * fs * (27 + 6 + 3) + copies
 
 
 def state_digest(solver):
