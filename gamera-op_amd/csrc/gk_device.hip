@@ -41,6 +41,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 #include <algorithm>
 
@@ -2477,6 +2478,7 @@ struct gk_solver {
   void *V[3];          // 2 face-velocity fields per direction
   void *E;             // 3 edge fields
   float *tables_f;     // FP32 copies of the reconstruction tables
+  double *pin;         // pinned host bounce buffer of one such field, lazily
   double *stage;       // host transfer staging field (reference layout), lazily
   int precision;       // 0 FP64, 1 FP32-bulk
   size_t esz;          // bytes per intermediate value
@@ -2766,6 +2768,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   (void)hipFree(s->tables_f);
   (void)hipFree(s->ring_tab);
   (void)hipFree(s->stage);
+  if (s->pin) (void)hipHostFree(s->pin);
   (void)hipFree(s->itables);
   for (void *p : s->field_mem) (void)hipFree(p);
   (void)hipFree(s->sc);
@@ -2802,17 +2805,43 @@ static int copy_in(gk_solver *s, double *dst, const double *src) {
   return GK_OK;
 }
 
+// Host copy of n doubles, split over a few threads for large fields (a
+// single thread copies pinned -> pageable memory at well under the PCIe rate).
+static void host_copy(double *dst, const double *src, size_t n) {
+  const unsigned hc = std::thread::hardware_concurrency();
+  const size_t nt = n < ((size_t)1 << 22) ? 1 : std::min<size_t>(8, hc ? hc : 1);
+  if (nt == 1) {
+    memcpy(dst, src, n * sizeof(double));
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (size_t t = 0; t < nt; t++) {
+    const size_t a = t * per, b = std::min(n, a + per);
+    if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, (b - a) * sizeof(double)); });
+  }
+  for (std::thread &x : th) x.join();
+}
+
+// Device -> host: re-pitch into the staging field, copy it to a pinned
+// bounce buffer (full PCIe rate), then into the caller's array.
 static int copy_out(gk_solver *s, double *dst, const double *src) {
   double *st;
   int rc = staging(s, &st);
   if (rc != GK_OK) return rc;
   const long rows = (long)s->g.N[0] * s->g.N[1];
   const int NK = s->g.N[2];
+  const size_t n = (size_t)rows * NK;
+  if (!s->pin && hipHostMalloc((void **)&s->pin, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    s->pin = nullptr;
+    return GK_ERR_NOMEM;
+  }
   k_pitch<<<2048, 256, 0, s->stream>>>(src, st, rows, NK, (int)s->g.st[1], NK);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(dst, st, rows * NK * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  // the staging field is reused by the next copy: finish this one first
+  HIPCHK(hipMemcpyAsync(s->pin, st, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  // the staging field and the bounce buffer are reused by the next copy
   HIPCHK(hipStreamSynchronize(s->stream));
+  host_copy(dst, s->pin, n);
   return GK_OK;
 }
 
