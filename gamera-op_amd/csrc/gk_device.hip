@@ -424,6 +424,21 @@ struct WorkItem {
   int arg, bx, by, bz;
 };
 
+// Marching kernels split the march range [lo, hi] into nch near-equal chunks
+// (lengths differ by at most one), one per thread; chunk bz is [f0, f1).
+// Equal shares avoid a nearly empty last chunk (257 faces at 32 per thread
+// had left a one-face chunk in every column).
+__device__ __forceinline__ void march_range(int lo, int hi, int nch, int bz, int &f0, int &f1) {
+  const long nf = (long)hi - lo + 1;
+  f0 = lo + (int)(bz * nf / nch);
+  f1 = lo + (int)((bz + 1) * nf / nch);
+}
+
+// chunks for a march range of nf points at about `per` points per thread
+static inline int march_chunks(int nf, int per) {
+  return std::max(1, std::min(nf, (nf + per / 2) / per));
+}
+
 // ---------------------------------------------------------------------------
 struct SweepArgs {
   Grid g;
@@ -439,7 +454,7 @@ struct SweepArgs {
   const Scal *sc;
   int lo[3], hi[3];
   int flo[3], fhi[3];    // range where the full flux is needed
-  int chunk;             // faces per thread in the marching sweeps
+  int nch;               // marching sweeps: near-equal chunks of the march range
 };
 
 // Direct sweep for the velocity-only strips of the j sweep (k in the four
@@ -689,14 +704,13 @@ __device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R 
 template <int D, typename R>
 __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
-  const int chunk = a.chunk;
   constexpr int RA = D == 0 ? 1 : 0;  // row axis
   const Grid &g = a.g;
   (void)by;
   int k, r;
   if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
-  const int f0 = a.lo[D] + bz * chunk;
-  const int f1 = min(f0 + chunk, a.hi[D] + 1);
+  int f0, f1;
+  march_range(a.lo[D], a.hi[D], a.nch, bz, f0, f1);
   int idx[3];
   idx[2] = k;
   idx[RA] = r;
@@ -998,7 +1012,7 @@ struct EmfArgs {
   void *E;        // 1 field of R
   const Scal *sc;
   int lo[3], hi[3];
-  int chunk;      // edges per thread along the march axis
+  int nch;        // near-equal chunks of the march range (one per thread)
 };
 
 // Marching EMF along the axis MA that two of the three reconstructions share
@@ -1089,7 +1103,6 @@ struct EmfWaves {
 template <int D, typename R, bool STAGED = false>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
-  const int chunk = a.chunk;
   constexpr int MA = D == 1 ? 0 : 1;  // march axis
   constexpr int RA = D == 1 ? 1 : 0;  // row axis
   constexpr int A2 = D == 0 ? 2 : (D == 1 ? 0 : 1);  // velocity (and F1) axis
@@ -1110,8 +1123,8 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
     (void)by;
     if (!row_point(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA], bx, k, r)) return;
   }
-  const int f0 = a.lo[MA] + bz * chunk;
-  const int f1 = min(f0 + chunk, a.hi[MA] + 1);
+  int f0, f1;
+  march_range(a.lo[MA], a.hi[MA], a.nch, bz, f0, f1);
   int idx[3];
   idx[2] = k;
   idx[RA] = r;
@@ -3192,7 +3205,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.F = s->F[D];
     a.V = s->V[D];
     a.sc = s->sc;
-    a.chunk = ch;
+    a.nch = march_chunks(Ee[D] + 1 - S[D] + 1, ch);
     for (int d = 0; d < 3; d++) {
       a.lo[d] = S[d];
       a.hi[d] = Ee[d];
@@ -3208,8 +3221,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.hi[X2] = g.N[X2] - 2;
     if (D == 0) {
       L.kid = SK_SWEEP_I;
-      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[1], a.hi[1]), 1,
-                    (a.hi[0] - a.lo[0] + ch) / ch);
+      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[1], a.hi[1]), 1, a.nch);
       L.u.sw = a;
       out.push_back(L);
     } else if (D == 1) {
@@ -3217,8 +3229,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       m.lo[2] = S[2];
       m.hi[2] = Ee[2];
       L.kid = SK_SWEEP_J;
-      L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1,
-                    (m.hi[1] - m.lo[1] + ch) / ch);
+      L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1, m.nch);
       L.u.sw = m;
       out.push_back(L);
       // velocity-only strips k in [0, ks-1] and [ke+1, NK-2]
@@ -3269,14 +3280,13 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.V = s->V[D == 0 ? 1 : (D == 1 ? 2 : 0)];
     a.E = (char *)s->E + (size_t)D * s->fs * s->esz;
     a.sc = s->sc;
-    a.chunk = s->emf_chunk;
     for (int d = 0; d < 3; d++) {
       a.lo[d] = S[d];
       a.hi[d] = Ee[d] + (d == D ? 0 : 1);
     }
     const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
-    const int ce = s->emf_chunk;
-    const int nf = (a.hi[MA] - a.lo[MA] + ce) / ce;
+    a.nch = march_chunks(a.hi[MA] - a.lo[MA] + 1, s->emf_chunk);
+    const int nf = a.nch;
     // FP64 E_i: whole 64-edge tiles along k in the staged form, the rest (all
     // of it when there is no whole tile) in the marching form.  Staging
     // measured slower for E_j (one direct field) and for the FP32 form.
@@ -3291,12 +3301,11 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       a.lo[2] = t.hi[2] + 1;
       // the leftover columns are few points: march one edge per thread so
       // that they spread over many waves
-      a.chunk = 1;
+      a.nch = a.hi[MA] - a.lo[MA] + 1;
     }
     if (a.lo[2] <= a.hi[2]) {
       L.kid = SK_EMF_I + D;
-      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA]), 1,
-                    (a.hi[MA] - a.lo[MA] + a.chunk) / a.chunk);
+      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[RA], a.hi[RA]), 1, a.nch);
       L.u.em = a;
       out.push_back(L);
     }
