@@ -157,9 +157,11 @@ int gk_debug_fetch(gk_solver *s, int which, int field, double *host);
 size_t gk_device_bytes(const gk_solver *s);
 
 /* Launch-shape parameters (no counterpart in the reference; they change
-   speed, never results).  GK_TUNE_CHUNK: faces each thread walks in the
-   i and j sweeps (1..4096, default 32); GK_TUNE_EMF_CHUNK: edges each thread
-   walks in the edge-field kernels (1..4096, default 16). */
+   speed, never results).  GK_TUNE_CHUNK: about how many faces each thread
+   walks in the i and j sweeps (1..4096, default 32); GK_TUNE_EMF_CHUNK:
+   about how many edges each thread walks in the edge-field kernels
+   (1..4096, default 16).  A column is split into near-equal chunks of
+   about that length (within 1.5x of it). */
 enum { GK_TUNE_CHUNK = 0, GK_TUNE_EMF_CHUNK = 1 };
 int gk_set_tuning(gk_solver *s, int param, int value);
 int gk_get_tuning(const gk_solver *s, int param);
