@@ -3228,6 +3228,15 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       SweepArgs m = a;  // full faces: k over the active range
       m.lo[2] = S[2];
       m.hi[2] = Ee[2];
+      // on grids that still fill every wave slot of the march (1024 SIMDs
+      // x 64 lanes x its waves per SIMD: one in FP64, two in FP32), the j
+      // march walks twice the i march's length: there its prologue costs
+      // more than the longer march loses in parallelism (256^3: 64 faces per
+      // thread fastest for j, 32 for i, in both forms); smaller grids
+      // stepped alone keep the parallelism
+      const long cols = (long)(m.hi[2] - m.lo[2] + 1) * (m.hi[0] - m.lo[0] + 1);
+      const int n2 = march_chunks(m.hi[1] - m.lo[1] + 1, std::min(2 * ch, 4096));
+      if (cols * n2 >= 1024L * 64 * (f32 ? 2 : 1)) m.nch = n2;
       L.kid = SK_SWEEP_J;
       L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1, m.nch);
       L.u.sw = m;
