@@ -158,7 +158,8 @@ size_t gk_device_bytes(const gk_solver *s);
 
 /* Launch-shape parameters (no counterpart in the reference; they change
    speed, never results).  GK_TUNE_CHUNK: about how many faces each thread
-   walks in the i and j sweeps (1..4096, default 32); GK_TUNE_EMF_CHUNK:
+   walks in the i and j sweeps (1..4096, default 32; the j sweep walks twice
+   that on grids large enough to fill the GPU with it); GK_TUNE_EMF_CHUNK:
    about how many edges each thread walks in the edge-field kernels
    (1..4096, default 16).  A column is split into near-equal chunks of
    about that length (within 1.5x of it). */
