@@ -200,9 +200,6 @@ def run_ring(args):
             "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]}}
 
 
-ENSEMBLE_SIZES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 32)]
-
-
 def run_ensemble(args):
     """Ensemble of independent Cartesian periodic grids (sizes drawn by seed
     from ENSEMBLE_SIZES; the sinusoid generator with its own seed, half of
@@ -211,15 +208,10 @@ def run_ensemble(args):
     autotuned march lengths) against the same grids stepped one after the
     other."""
     sys.path.insert(0, PKG)
-    import numpy as np
     import gamera_op as go
+    from ensemble import ENSEMBLE_SIZES, ensemble_specs
 
-    rng = np.random.default_rng(1000 + args.seed)
-    specs = []
-    for q in range(args.ens_instances):
-        shape = ENSEMBLE_SIZES[rng.integers(len(ENSEMBLE_SIZES))]
-        kind = go.SINUSOID_SPHERE if rng.random() < 0.5 else go.SINUSOID
-        specs.append((kind, int(rng.integers(1, 2**31)), shape))
+    specs = ensemble_specs(args.ens_instances, 1000 + args.seed)
     log(f"bench: ensemble of {len(specs)} grids, set-up")
     solvers = []
     cells = 0
@@ -269,8 +261,12 @@ def run_serving(args, preparer):
     reqs = serve.make_requests(args.serve_requests, args.serve_rate, seed=args.seed)
     log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, budget "
         f"{args.serve_budget_gib} GiB")
+    # launch shapes of the mix's grid shapes, tuned before the stream starts
+    tuning = serve.tune_shapes(reqs)
+    log(f"bench: tuned the mix's shapes in {tuning:.1f} s")
     res, wall = serve.serve(reqs, budget_bytes=args.serve_budget_gib << 30, preparer=preparer)
     srv = serve.summarize(res, wall)
+    srv["tuning_s_before_stream"] = round(tuning, 3)
     srv["loop_seconds"] = {k: round(v, 3) for k, v in serve.serve.last_stats.items()}
     log(f"bench: served {srv}")
     log("bench: plain loop on the same requests")

@@ -52,6 +52,8 @@ namespace {
 constexpr int NGH = 4;
 constexpr double GAMMA = 5.0 / 3.0;
 constexpr double CFL = 0.3;
+// initial values of the reference's positivity floors (src/config.c:89-90);
+// a problem may change them, so the cell update takes them as arguments
 constexpr double RHO_FLOOR = 0.00316227;
 constexpr double P_FLOOR = 0.000316227;
 
@@ -86,13 +88,28 @@ struct B0c {
 constexpr int NSLOT = 64;
 constexpr unsigned long long DBL_INF_BITS = 0x7FF0000000000000ULL;
 
+// Run control: a call arms the grid (k_arm) with a step budget and two stop
+// times; k_begin stops the grid after the step that spends the budget or
+// whose time exceeds t_stop (the reference's time_stop test,
+// src/solver_solve.c:523-527) or reaches t_event (a dump or status line is
+// due), and every kernel of a stopped grid's later steps returns at once, so
+// a run of steps needs no host round trip to end where the reference's loop
+// would pause.
 struct Scal {
   double dt, dt0, time, c;
   unsigned long long smin[NSLOT];  // bit images of partial minima of the next CFL dt (>= 0)
   unsigned long long smax[NSLOT];  // bit images of partial maxima of |div B|
   unsigned int nanflag;
+  unsigned int halt;     // this step's kernels do nothing
+  unsigned int stop;     // the next step halts
   unsigned int pad;
+  long long budget;      // steps the grid may still take (< 0: no limit)
+  long long steps;       // steps taken since the grid was armed
+  double t_stop, t_event;
 };
+
+// every stage kernel of a halted step returns at once (null: not a step)
+__device__ __forceinline__ bool halted(const Scal *sc) { return sc && sc->halt; }
 
 __device__ __forceinline__ double geo3(const Geo &g, int q, int i, int j, int k) {
   return g.g[q][0][i] * g.g[q][1][j] * g.g[q][2][k];
@@ -165,6 +182,11 @@ __device__ __forceinline__ void block_reduce(double dtc, double divb, Scal *sc, 
 // the AB2 factor and time, dt0 := dt, and reset the slots.
 __device__ __forceinline__ void begin_step(Scal *sc) {
   const int q = threadIdx.x;  // NSLOT == 64 lanes
+  if (sc->halt) return;
+  if (sc->stop) {  // the previous step ended the run: this step and later ones halt
+    if (q == 0) sc->halt = 1;
+    return;
+  }
   const double mn = wave_min(__longlong_as_double((long long)sc->smin[q]));
   sc->smin[q] = DBL_INF_BITS;
   sc->smax[q] = 0ULL;
@@ -174,6 +196,9 @@ __device__ __forceinline__ void begin_step(Scal *sc) {
     sc->time += dt;
     sc->c = 0.5 * dt / sc->dt0;
     sc->dt0 = dt;
+    sc->steps += 1;
+    if (sc->budget > 0) sc->budget -= 1;
+    if (sc->budget == 0 || sc->time > sc->t_stop || sc->time >= sc->t_event) sc->stop = 1;
   }
 }
 
@@ -181,6 +206,24 @@ __global__ __launch_bounds__(64) void k_begin(Scal *sc) { begin_step(sc); }
 
 __global__ __launch_bounds__(64) void k_begin_batch(Scal *const *scs, int n) {
   if ((int)blockIdx.x < n) begin_step(scs[blockIdx.x]);
+}
+
+// arm a grid for a run: step budget (< 0: none) and stop times
+__global__ __launch_bounds__(64) void k_arm(Scal *sc, long long budget, double t_stop,
+                                            double t_event) {
+  if (threadIdx.x == 0) {
+    sc->budget = budget;
+    sc->steps = 0;
+    sc->t_stop = t_stop;
+    sc->t_event = t_event;
+    sc->halt = 0;
+    sc->stop = budget == 0 ? 1u : 0u;
+  }
+}
+
+// non-finite flags of the grids of a batch (one word each)
+__global__ __launch_bounds__(64) void k_flags(Scal *const *scs, int n, unsigned int *out) {
+  for (int q = threadIdx.x; q < n; q += 64) out[q] = scs[q]->nanflag;
 }
 
 // dt0 := CFL dt of the initial state (prologue of a fresh start)
@@ -464,6 +507,7 @@ struct SweepArgs {
 template <int D, typename R>
 __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
+  if (halted(a.sc)) return;
   (void)by;
   (void)bz;
   const int W = a.hi[2] - a.lo[2] + 1, H = a.hi[1] - a.lo[1] + 1;
@@ -546,6 +590,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
 template <typename R>
 __device__ __forceinline__ void k_kcol_body(const SweepArgs &a, const int bx, const int by,
                                        const int bz) {
+  if (halted(a.sc)) return;
   (void)by;
   (void)bz;
   __shared__ R sx[64][9], sh[64][9];  // [row][cell], padded against bank conflicts
@@ -704,6 +749,7 @@ __device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R 
 template <int D, typename R>
 __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
+  if (halted(a.sc)) return;
   constexpr int RA = D == 0 ? 1 : 0;  // row axis
   const Grid &g = a.g;
   (void)by;
@@ -844,6 +890,7 @@ constexpr int CELL_CHUNK = 1;  // cells each thread of the cell update walks alo
 template <typename R>
 __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   __shared__ R sh[9][KSPAN];
   __shared__ R sx[9][KSPAN];
   const Grid &g = a.g;
@@ -1103,6 +1150,7 @@ struct EmfWaves {
 template <int D, typename R, bool STAGED = false>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
+  if (halted(a.sc)) return;
   constexpr int MA = D == 1 ? 0 : 1;  // march axis
   constexpr int RA = D == 1 ? 1 : 0;  // row axis
   constexpr int A2 = D == 0 ? 2 : (D == 1 ? 0 : 1);  // velocity (and F1) axis
@@ -1333,6 +1381,7 @@ struct CtArgs {
 template <typename R>
 __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const int by,
                                      const int bz) {
+  if (halted(a.sc)) return;
   const Grid &g = a.g;
   (void)by;
   int k, j;
@@ -1416,6 +1465,8 @@ struct CellArgs {
   float *Mn32;             // its FP32 copy (FP32-bulk form), else null
   const double *Bn;        // face B at n+1
   const void *F[3];        // 9 fields of R per direction
+  double rho_floor, p_floor;  // positivity floors
+  double grav[3];          // uniform gravity G01..G03
   Scal *sc;
   int skip[4];             // j rows [skip[0], skip[1]], [skip[2], skip[3]] left out of
                            // the fused dt / div B reduction (pole ring rows,
@@ -1429,6 +1480,7 @@ struct CellArgs {
 template <typename R>
 __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, const int by,
                                        const int bz) {
+  if (halted(a.sc)) return;
   const Grid &g = a.g;
   const Geo &G = a.geo;
   int k, j;
@@ -1556,11 +1608,11 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     double p = (eng - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) * rho) * (GAMMA - 1.0);
     double ps = entropy_times(ent, rho);
     double rhof = rho;
-    if (rhof < RHO_FLOOR) {
-      rhof = RHO_FLOOR;
-      p = P_FLOOR;
+    if (rhof < a.rho_floor) {
+      rhof = a.rho_floor;
+      p = a.p_floor;
     }
-    if (p < P_FLOOR) p = P_FLOOR;
+    if (p < a.p_floor) p = a.p_floor;
 
     // time-centred total field
     double mt[3];
@@ -1580,7 +1632,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     for (int m = 0; m < 3; m++) {
       double rv = ru0[m] + perp * ((-dt * dU[1 + m] + dt * SH[m]) + (-dt * dU[6 + m] + dt * SB[m]) +
                                    dva * x[1 + m] + mt[m] * bdv);
-      rv += dt * rho * 0.0;  // gravity G0 = 0 (src/config.c:95-97)
+      rv += dt * rho * a.grav[m];  // gravity (src/solver_solve.c:469-471)
       vn[m] = rv / rho;
     }
     double out[6] = {rhof, vn[0], vn[1], vn[2], p, ps};
@@ -1655,6 +1707,7 @@ struct DtArgs {
 
 __device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   const Grid &g = a.g;
   const Geo &G = a.geo;
   const int k = g.s[2] + bx * blockDim.x + threadIdx.x;
@@ -1718,6 +1771,7 @@ struct GhostArgs {
   float *X32;   // FP32 copies (FP32-bulk form), else null
   float *B32;
   int bc[2][6];
+  const Scal *sc;  // run control of a step's launch (null outside a step)
 };
 
 __device__ __forceinline__ bool mirror_kind(int kind) {
@@ -1808,6 +1862,7 @@ __device__ __forceinline__ bool ghost_point(const Grid &g, int bx, int &i, int &
 
 __device__ __forceinline__ void k_ghost_body(const GhostArgs &a, const int bx, const int by,
                                         const int bz) {
+  if (halted(a.sc)) return;
   (void)by;
   (void)bz;
   const Grid &g = a.g;
@@ -1942,11 +1997,13 @@ struct MagsArgs {
   const double *B;
   double *M;   // or null (not written)
   float *M32;  // FP32 copy (FP32-bulk form), else null
+  const Scal *sc;  // run control of a step's launch (null outside a step)
 };
 
 template <bool SHELL = false>
 __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
                                        const int bz) {
+  if (halted(a.sc)) return;
   const Grid &g = a.g;
   (void)by;
   int i, j, k;
@@ -2046,6 +2103,7 @@ struct RingArgs {
   int nk, rows;
   int pole[2];        // theta = 0 / theta = pi is a pole
   int nchunk[16];     // blocks of ring row r (0 at the pole)
+  const Scal *sc;     // run control
 };
 
 __device__ __forceinline__ double ring_fsign(double x) {
@@ -2095,6 +2153,7 @@ __device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, doubl
 template <typename R>
 __device__ __forceinline__ void k_ring_rows_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   __shared__ double q[RING_N];
   __shared__ double blk[RING_B];
   __shared__ double co[3];
@@ -2217,6 +2276,7 @@ __device__ __forceinline__ double ring_mean(const T *v, int n, double *lds) {
 template <typename R>
 __device__ __forceinline__ void k_pole_e_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   (void)bz;  // 2-D launch
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
@@ -2252,6 +2312,7 @@ __global__ __launch_bounds__(64) void k_pole_e_batch(const RingArgs *__restrict_
 
 __device__ __forceinline__ void k_pole_b_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
@@ -2313,6 +2374,7 @@ __device__ __forceinline__ double ring_ej(const RingArgs &a, int i, int j, int k
 template <typename R>
 __device__ __forceinline__ void k_ring_curl_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
@@ -2376,6 +2438,7 @@ __global__ __launch_bounds__(256) void k_ring_curl_batch(const RingArgs *__restr
 // over the nk + 1 faces of the ring.  One wave per (pole, i).
 __device__ __forceinline__ void k_ring_fix_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   (void)bz;  // 2-D launch
   __shared__ double lds[RING_N + 2];
   const Grid &g = a.g;
@@ -2420,6 +2483,7 @@ __global__ __launch_bounds__(64) void k_ring_fix_batch(const RingArgs *__restric
 // pass computes it (src/solver_func.c:310-322)
 __device__ __forceinline__ void k_ring_mags_body(const RingArgs &a, const int bx, const int by,
                                          const int bz) {
+  if (halted(a.sc)) return;
   (void)bz;  // 2-D launch
   const Grid &g = a.g;
   const int p = by;
@@ -2492,6 +2556,7 @@ struct gk_solver {
   void *E;             // 3 edge fields
   float *tables_f;     // FP32 copies of the reconstruction tables
   double *pin;         // pinned host bounce buffer of one such field, lazily
+  long long *ctl_host; // pinned: steps taken, non-finite flag of the last run
   double *stage;       // host transfer staging field (reference layout), lazily
   int precision;       // 0 FP64, 1 FP32-bulk
   size_t esz;          // bytes per intermediate value
@@ -2502,7 +2567,10 @@ struct gk_solver {
   int timing;
   int chunk;           // faces per thread in the marching sweeps
   int emf_chunk;       // edges per thread in the marching EMF kernels
+  double rho_floor, p_floor;  // positivity floors
+  double grav[3];      // uniform gravity
   struct gk_batch *batch;  // batch that owns the stepping, if any
+  long long budget;    // batch member: steps it may still take (< 0: no limit)
   int ring;            // pole ring averaging on
   int ring_pole[2];
   int ring_rows;
@@ -2559,6 +2627,13 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if (d->coord != 0 && d->coord != 2) return GK_ERR_ARG;
   if (d->precision != 0 && d->precision != 1) return GK_ERR_ARG;
   if (d->b0_mode != 0 && d->b0_mode != 1) return GK_ERR_ARG;
+  // floors: >= 0 and finite (0 selects the reference's initial value);
+  // gravity finite
+  if (!(d->rho_floor >= 0.0 && d->rho_floor < INFINITY) ||
+      !(d->p_floor >= 0.0 && d->p_floor < INFINITY))
+    return GK_ERR_ARG;
+  for (int m = 0; m < 3; m++)
+    if (!std::isfinite(d->gravity[m])) return GK_ERR_ARG;
   for (int q = 0; q < GK_G_COUNT; q++)
     for (int a = 0; a < 3; a++)
       if (!d->geom[q][a]) return GK_ERR_ARG;
@@ -2626,6 +2701,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   s->precision = d->precision;
   s->esz = d->precision == 1 ? sizeof(float) : sizeof(double);
   memcpy(s->bc, d->bc, sizeof(s->bc));
+  s->rho_floor = d->rho_floor > 0.0 ? d->rho_floor : RHO_FLOOR;
+  s->p_floor = d->p_floor > 0.0 ? d->p_floor : P_FLOOR;
+  for (int m = 0; m < 3; m++) s->grav[m] = d->gravity[m];
+  s->budget = -1;
   s->b0 = B0c{};
   if (d->b0_mode == 1) {
     memcpy(s->b0.cell, d->b0_cell, sizeof(s->b0.cell));
@@ -2759,7 +2838,15 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
   if ((rc = field_alloc(s, &s->E, 3 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
   if (hipMalloc((void **)&s->sc, sizeof(Scal)) != hipSuccess) return fail(GK_ERR_NOMEM);
+  if (hipHostMalloc((void **)&s->ctl_host, 2 * sizeof(long long), hipHostMallocDefault) !=
+      hipSuccess) {
+    s->ctl_host = nullptr;
+    return fail(GK_ERR_NOMEM);
+  }
   Scal z{};
+  z.budget = -1;
+  z.t_stop = INFINITY;
+  z.t_event = INFINITY;
   if (hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream) != hipSuccess)
     return fail(GK_ERR_DEVICE);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
@@ -2782,6 +2869,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   (void)hipFree(s->ring_tab);
   (void)hipFree(s->stage);
   if (s->pin) (void)hipHostFree(s->pin);
+  if (s->ctl_host) (void)hipHostFree(s->ctl_host);
   (void)hipFree(s->itables);
   for (void *p : s->field_mem) (void)hipFree(p);
   (void)hipFree(s->sc);
@@ -2906,6 +2994,7 @@ static void launch_ghost(gk_solver *s, double *X, double *B) {
   a.X32 = nullptr;
   a.B32 = nullptr;
   memcpy(a.bc, s->bc, sizeof(a.bc));
+  a.sc = nullptr;
   const dim3 gr = ghost_grid(s->g);
   const int tot = gr.x * gr.y * gr.z;
   k_ghost<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
@@ -2918,6 +3007,7 @@ static void launch_mags(gk_solver *s, const double *B, double *M) {
   a.B = B;
   a.M = M;
   a.M32 = nullptr;
+  a.sc = nullptr;
   const dim3 gr = mags_grid(s->g);
   const int tot = patch_blocks(gr);
   k_mags<<<dim3((tot + NXCD - 1) / NXCD * NXCD), dim3(64, 4), 0, s->stream>>>(a, gr.x, gr.y, gr.z);
@@ -3008,6 +3098,9 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
     z.smax[q] = 0ULL;
   }
   z.time = cur.time;  // keep the simulated time across prologues
+  z.budget = -1;
+  z.t_stop = INFINITY;
+  z.t_event = INFINITY;
   HIPCHK(hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
   DtArgs a;
   a.g = s->g;
@@ -3338,6 +3431,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     ra.pole[0] = s->ring_pole[0];
     ra.pole[1] = s->ring_pole[1];
     memcpy(ra.nchunk, s->ring_nchunk, sizeof(ra.nchunk));
+    ra.sc = s->sc;
     L.kid = SK_POLE_E;  // E_i ring mean and E_k := 0 on the pole edges
     L.grid = dim3(s->ni + 1, 2, 1);
     L.u.rg = ra;
@@ -3383,6 +3477,9 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.Mn32 = f32 ? s->M32[r.b_n] : nullptr;
     a.Bn = Bn;
     for (int d = 0; d < 3; d++) a.F[d] = s->F[d];
+    a.rho_floor = s->rho_floor;
+    a.p_floor = s->p_floor;
+    for (int m = 0; m < 3; m++) a.grav[m] = s->grav[m];
     a.sc = s->sc;
     a.skip[0] = a.skip[2] = 1;  // empty ranges
     a.skip[1] = a.skip[3] = 0;
@@ -3431,6 +3528,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.X32 = f32 ? s->X32[1 - r.cur] : nullptr;
     a.B32 = f32 ? s->B32[r.b_n] : nullptr;
     memcpy(a.bc, s->bc, sizeof(a.bc));
+    a.sc = s->sc;
     L.kid = SK_GHOST;
     L.grid = ghost_grid(g);
     L.u.gh = a;
@@ -3444,6 +3542,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.B = Bn;
     a.M = f32 ? nullptr : Mn;  // FP32-bulk: only the FP32 copy is read
     a.M32 = f32 ? s->M32[r.b_n] : nullptr;
+    a.sc = s->sc;
     L.kid = SK_MAGS;  // ghost cells only (k_mags_shell)
     L.grid = mags_shell_grid(g);
     L.u.mg = a;
@@ -3556,15 +3655,19 @@ static void one_step(gk_solver *s) {
   s->b_n = r.b_n;
 }
 
-extern "C" int gk_step(gk_solver *s, int n) {
-  if (!s) return GK_ERR_ARG;
-  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
-  if (n < 0) return GK_ERR_ARG;
+// Arm the grid, enqueue n steps and wait for them; the device stops the run
+// early when the budget or a stop time is reached (later steps of the
+// enqueued run then do nothing).  *done receives the steps taken; the host
+// history rotation follows them.
+static int run_steps(gk_solver *s, int n, long long budget, double t_stop, double t_event,
+                     int *done) {
   if (s->timing)
     for (size_t q = 0; q < s->acc_ms.size(); q++) {
       s->acc_ms[q] = 0.0;
       s->acc_n[q] = 0;
     }
+  hipLaunchKernelGGL(k_arm, dim3(1), dim3(64), 0, s->stream, s->sc, budget, t_stop, t_event);
+  const Rot r0 = rot_of(s);
   for (int t = 0; t < n; t++) {
     one_step(s);
     if (s->timing && s->pend.size() >= 4096) {  // bound the event pool
@@ -3573,9 +3676,41 @@ extern "C" int gk_step(gk_solver *s, int n) {
     }
   }
   HIPCHK(hipGetLastError());
+  // steps taken and the non-finite flag, read after the run drains
+  HIPCHK(hipMemcpyAsync(s->ctl_host, &s->sc->steps, sizeof(long long), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipMemcpyAsync(s->ctl_host + 1, &s->sc->nanflag, sizeof(unsigned int),
+                        hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (s->timing) resolve_timers(s);
-  return GK_OK;
+  long long took = 0;
+  unsigned int bad = 0;
+  memcpy(&took, s->ctl_host, sizeof took);
+  memcpy(&bad, s->ctl_host + 1, sizeof bad);
+  if (took < n) {  // the run stopped early: the rotation follows the steps taken
+    Rot r = r0;
+    for (long long t = 0; t < took % 6; t++) r = rot_next(r);
+    s->cur = r.cur;
+    s->b_m1 = r.b_m1;
+    s->b_0 = r.b_0;
+    s->b_n = r.b_n;
+  }
+  if (done) *done = (int)took;
+  return bad ? GK_ERR_NAN : GK_OK;
+}
+
+extern "C" int gk_step(gk_solver *s, int n) {
+  if (!s) return GK_ERR_ARG;
+  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
+  if (n < 0) return GK_ERR_ARG;
+  return run_steps(s, n, -1, INFINITY, INFINITY, nullptr);
+}
+
+extern "C" int gk_step_until(gk_solver *s, int n, double t_stop, double t_event, int *done) {
+  if (!s) return GK_ERR_ARG;
+  if (s->ready != 3 || s->batch) return GK_ERR_STATE;
+  if (n < 0 || t_stop != t_stop || t_event != t_event) return GK_ERR_ARG;
+  return run_steps(s, n, -1, t_stop, t_event, done);
 }
 
 extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], double *prim_prev[6],
@@ -3714,6 +3849,7 @@ extern "C" int gk_copy_calibrate(size_t bytes, int reps, double *ms) {
 extern "C" int gk_set_tuning(gk_solver *s, int param, int value) {
   if (!s) return GK_ERR_ARG;
   if (value < 1 || value > 4096) return GK_ERR_ARG;
+  if (s->batch) return GK_ERR_STATE;  // a batch's launch tables hold its shapes
   if (param == GK_TUNE_CHUNK) {
     s->chunk = value;
     return GK_OK;
@@ -3755,6 +3891,12 @@ struct gk_batch {
   std::vector<std::string> names;
   std::vector<double> acc_ms;
   std::vector<int> acc_n;
+  // non-finite flags of the members (member order), gathered after each
+  // launch of steps into pinned host memory
+  Scal **dscs = nullptr;
+  unsigned int *dflags = nullptr;
+  unsigned int *hflags = nullptr;
+  int nflags = 0;
 };
 
 static void batch_free_tables(gk_batch *b) {
@@ -3769,8 +3911,15 @@ static void batch_free_tables(gk_batch *b) {
   }
   (void)hipFree(b->dargs);
   (void)hipFree(b->dwork);
+  (void)hipFree(b->dscs);
+  (void)hipFree(b->dflags);
+  if (b->hflags) (void)hipHostFree(b->hflags);
   b->dargs = nullptr;
   b->dwork = nullptr;
+  b->dscs = nullptr;
+  b->dflags = nullptr;
+  b->hflags = nullptr;
+  b->nflags = 0;
 }
 
 static int batch_build(gk_batch *b) {
@@ -3823,6 +3972,22 @@ static int batch_build(gk_batch *b) {
         HIPCHK(hipMemcpy(b->dwork + b->work_off[gp][k], hwork[gp][k].data(),
                          hwork[gp][k].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     }
+  const int nm = (int)b->mem.size();
+  if (nm) {
+    std::vector<Scal *> hs(nm);
+    for (int q = 0; q < nm; q++) hs[q] = b->mem[q]->sc;
+    if (hipMalloc((void **)&b->dscs, nm * sizeof(Scal *)) != hipSuccess ||
+        hipMalloc((void **)&b->dflags, nm * sizeof(unsigned int)) != hipSuccess)
+      return GK_ERR_NOMEM;
+    if (hipHostMalloc((void **)&b->hflags, nm * sizeof(unsigned int), hipHostMallocDefault) !=
+        hipSuccess) {
+      b->hflags = nullptr;
+      return GK_ERR_NOMEM;
+    }
+    memset(b->hflags, 0, nm * sizeof(unsigned int));
+    HIPCHK(hipMemcpy(b->dscs, hs.data(), nm * sizeof(Scal *), hipMemcpyHostToDevice));
+    b->nflags = nm;
+  }
   b->phase = 0;
   b->dirty = false;
   return GK_OK;
@@ -3909,10 +4074,37 @@ extern "C" int gk_batch_add(gk_batch *b, gk_solver *s) {
   if (!b || !s) return GK_ERR_ARG;
   if (s->ready != 3 || s->batch) return GK_ERR_STATE;
   HIPCHK(hipStreamSynchronize(s->stream));
+  // a member steps without a budget until gk_batch_set_budget gives it one
+  hipLaunchKernelGGL(k_arm, dim3(1), dim3(64), 0, b->stream, s->sc, -1LL, (double)INFINITY,
+                     (double)INFINITY);
+  HIPCHK(hipGetLastError());
+  s->budget = -1;
   b->mem.push_back(s);
   s->batch = b;
   b->dirty = true;
   return GK_OK;
+}
+
+extern "C" int gk_batch_set_budget(gk_batch *b, gk_solver *s, long long steps) {
+  if (!b || !s || s->batch != b) return GK_ERR_ARG;
+  // ordered after the steps already launched: the budget counts from here
+  hipLaunchKernelGGL(k_arm, dim3(1), dim3(64), 0, b->stream, s->sc, steps < 0 ? -1LL : steps,
+                     (double)INFINITY, (double)INFINITY);
+  HIPCHK(hipGetLastError());
+  s->budget = steps < 0 ? -1 : steps;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_status(gk_batch *b, gk_solver *s) {
+  if (!b || !s || s->batch != b) return GK_ERR_ARG;
+  for (int q = 0; q < (int)b->mem.size() && q < b->nflags; q++)
+    if (b->mem[q] == s) return b->hflags[q] ? GK_ERR_NAN : GK_OK;
+  return GK_OK;
+}
+
+extern "C" long long gk_batch_budget(gk_batch *b, gk_solver *s) {
+  if (!b || !s || s->batch != b) return GK_ERR_ARG;
+  return s->budget;
 }
 
 extern "C" int gk_batch_remove(gk_batch *b, gk_solver *s) {
@@ -3994,10 +4186,25 @@ extern "C" int gk_batch_step_async(gk_batch *b, int n) {
       b->phase = (b->phase + 1) % 6;
     }
   }
-  // every grid's history rotation advances by n
+  // non-finite flags of every member, read back with the run
+  if (b->nflags) {
+    hipLaunchKernelGGL(k_flags, dim3(1), dim3(64), 0, b->stream, b->dscs, b->nflags, b->dflags);
+    HIPCHK(hipMemcpyAsync(b->hflags, b->dflags, b->nflags * sizeof(unsigned int),
+                          hipMemcpyDeviceToHost, b->stream));
+  }
+  // every grid's history rotation advances by the steps it takes: n, or
+  // what is left of its budget (the device stops it there; its later
+  // launches do nothing).  A member that stopped short is out of phase with
+  // the batch's argument tables, which are rebuilt before the next launch.
   for (gk_solver *s : b->mem) {
+    long long took = n;
+    if (s->budget >= 0) {
+      took = std::min<long long>(n, s->budget);
+      s->budget -= took;
+    }
+    if (took != n) b->dirty = true;
     Rot r = rot_of(s);
-    for (int t = 0; t < n % 6; t++) r = rot_next(r);
+    for (long long t = 0; t < took % 6; t++) r = rot_next(r);
     s->cur = r.cur;
     s->b_m1 = r.b_m1;
     s->b_0 = r.b_0;
@@ -4010,6 +4217,8 @@ extern "C" int gk_batch_step_async(gk_batch *b, int n) {
 extern "C" int gk_batch_sync(gk_batch *b) {
   if (!b) return GK_ERR_ARG;
   HIPCHK(hipStreamSynchronize(b->stream));
+  for (int q = 0; q < b->nflags; q++)
+    if (b->hflags[q]) return GK_ERR_NAN;
   return GK_OK;
 }
 
@@ -4036,7 +4245,7 @@ struct TunePick {
 static std::vector<std::pair<TuneKey, TunePick>> g_tune_cache;
 
 extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
-  if (!s || trial_steps < 1) return GK_ERR_ARG;
+  if (!s || trial_steps < 0) return GK_ERR_ARG;
   if (s->ready != 3 || s->batch) return GK_ERR_STATE;
   const TuneKey key{s->ni, s->nj, s->nk, s->spherical, s->precision};
   for (auto &e : g_tune_cache)
@@ -4046,6 +4255,10 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
       if (best) *best = e.second.chunk;
       return GK_OK;
     }
+  if (trial_steps == 0) {  // cache lookup only: no pick yet, nothing timed
+    if (best) *best = 0;
+    return GK_OK;
+  }
   const size_t fs = s->fs;
   const size_t nX = 12 * fs, nB = 9 * fs, nM = 9 * fs;
   double *save = nullptr;
@@ -4091,6 +4304,10 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   // fastest of three repeats, so that launch jitter does not decide the pick
   const int repeats = (long)s->ni * s->nj * s->nk <= (1L << 20) + (1L << 18) ? 3 : 1;
   auto trial = [&](float &ms) -> int {
+    // trials step without a budget or stop time (every stepping entry point
+    // arms the grid again afterwards)
+    hipLaunchKernelGGL(k_arm, dim3(1), dim3(64), 0, s->stream, s->sc, -1LL, (double)INFINITY,
+                       (double)INFINITY);
     one_step(s);
     ms = 1e30f;
     for (int rep = 0; rep < repeats; rep++) {  // back to back; one restore after

@@ -21,7 +21,10 @@ LIB_HOST = os.path.join(HERE, "libgamera_host.so")
 
 # seeded problem generators (gamera-op_amd/problems/gk_seeded.c)
 SINUSOID, SINUSOID_BLAST, DIPOLE_SHELL, SINUSOID_SPHERE, POLE_SPHERE, POLE_CAP = 0, 1, 2, 3, 4, 5
-WALL_BOX, INFLOW_BOX = 6, 7
+WALL_BOX, INFLOW_BOX, ENTROPY_BOX, FLOOR_BOX = 6, 7, 8, 9
+
+# gk_status codes (include/gamera_op.h)
+GK_OK, GK_ERR_ARG, GK_ERR_DEVICE, GK_ERR_NOMEM, GK_ERR_STATE, GK_ERR_NAN = 0, -1, -2, -3, -4, -5
 
 NF_GAS, NF_GEM, NF_GEO, NF_REC = 82, 97, 28, 42
 COORD_NAMES = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr",
@@ -86,7 +89,13 @@ def _load():
     h.gkh_reseed.argtypes = [ctypes.c_uint64]
     h.gkh_device_create.restype = ctypes.c_int
     h.gkh_device_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    _op.gk_batch_budget.restype = ctypes.c_longlong
+    _op.gk_batch_budget.argtypes = [vp, vp]
     for name, args in [("gk_autotune", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+                       ("gk_step_until", [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_int)]),
+                       ("gk_batch_set_budget", [vp, vp, ctypes.c_longlong]),
+                       ("gk_batch_status", [vp, vp]),
                        ("gk_destroy", [vp]),
                        ("gk_get_state", [vp] + [ctypes.POINTER(ctypes.c_void_p)] * 4),
                        ("gk_batch_create", [ctypes.POINTER(vp)]),
@@ -183,6 +192,10 @@ class GkError(RuntimeError):
     pass
 
 
+class GkNonFinite(GkError):
+    """The device state holds a non-finite value (GK_ERR_NAN)."""
+
+
 class Problem:
     """Host facade state for one seeded problem (global, one at a time)."""
 
@@ -256,9 +269,7 @@ class Problem:
         self.dev = self.h.gkh_device()
 
     def device_step(self, n):
-        rc = self.op.gk_step(self.dev, n)
-        if rc != 0:
-            raise GkError(f"gk_step failed ({rc})")
+        _check(self.op.gk_step(self.dev, n), "gk_step")
 
     def device_fetch(self):
         rc = self.h.gkh_device_fetch()
@@ -312,6 +323,8 @@ class Problem:
 
 
 def _check(rc, what):
+    if rc == GK_ERR_NAN:
+        raise GkNonFinite(f"{what}: non-finite state")
     if rc != 0:
         raise GkError(f"{what} failed ({rc})")
 
@@ -353,6 +366,14 @@ class Solver:
     def step(self, n):
         _check(self.op.gk_step(self.handle, n), "gk_step")
 
+    def step_until(self, n, t_stop=float("inf"), t_event=float("inf")):
+        """Up to n steps, ending after the first step whose time exceeds
+        t_stop or reaches t_event; returns the steps taken."""
+        done = ctypes.c_int(0)
+        _check(self.op.gk_step_until(self.handle, n, t_stop, t_event, ctypes.byref(done)),
+               "gk_step_until")
+        return done.value
+
     def scalars(self):
         out = (ctypes.c_double * 6)()
         _check(self.op.gk_get_scalars(self.handle, out), "gk_get_scalars")
@@ -384,6 +405,8 @@ class Solver:
         return {names[q].decode(): ms[q] for q in range(min(n.value, 32))}
 
     def autotune(self, trial_steps=3):
+        """Time the candidate launch shapes (trial_steps = 0: apply a cached
+        pick only; returns 0 when there is none)."""
         best = ctypes.c_int(0)
         _check(self.op.gk_autotune(self.handle, trial_steps, ctypes.byref(best)), "gk_autotune")
         return best.value
@@ -427,6 +450,25 @@ class Batch:
 
     def sync(self):
         _check(self.op.gk_batch_sync(self.handle), "gk_batch_sync")
+
+    def sync_status(self):
+        """Wait for the launched steps; returns the members whose state
+        holds a non-finite value (an empty list when there are none)."""
+        rc = self.op.gk_batch_sync(self.handle)
+        if rc == GK_ERR_NAN:
+            return [m for m in self.members
+                    if self.op.gk_batch_status(self.handle, m.handle) == GK_ERR_NAN]
+        _check(rc, "gk_batch_sync")
+        return []
+
+    def set_budget(self, solver, steps):
+        """The member takes `steps` more steps (counted from the steps
+        already launched), then stops on the device while the batch goes on."""
+        _check(self.op.gk_batch_set_budget(self.handle, solver.handle, steps),
+               "gk_batch_set_budget")
+
+    def budget(self, solver):
+        return int(self.op.gk_batch_budget(self.handle, solver.handle))
 
     def set_graphs(self, on):
         _check(self.op.gk_batch_set_graphs(self.handle, 1 if on else 0), "gk_batch_set_graphs")

@@ -302,6 +302,7 @@ typedef struct {
   int ni, nj, nk, coord, bc[2][6], b0_mode, N[3], ncls[3][2];
   int ring, ring_pole[2], ring_rows, ring_nchunk[16];
   double CA, b0_cell[3], b0_face[3][8], b0_edge[3][3], b0_lorentz[3];
+  double rho_floor, p_floor, gravity[3];
 } gk_blob_head;
 #define GK_BLOB_MAGIC 0x6b426c6fu
 
@@ -343,7 +344,7 @@ long gkh_desc_blob(void *buf, size_t cap) {
   gk_blob_head h;
   memset(&h, 0, sizeof h);
   h.magic = GK_BLOB_MAGIC;
-  h.version = 2;
+  h.version = 3;
   h.ring = d.ring;
   memcpy(h.ring_pole, d.ring_pole, sizeof h.ring_pole);
   h.ring_rows = d.ring_rows;
@@ -363,6 +364,9 @@ long gkh_desc_blob(void *buf, size_t cap) {
   memcpy(h.b0_face, d.b0_face, sizeof h.b0_face);
   memcpy(h.b0_edge, d.b0_edge, sizeof h.b0_edge);
   memcpy(h.b0_lorentz, d.b0_lorentz, sizeof h.b0_lorentz);
+  h.rho_floor = d.rho_floor;
+  h.p_floor = d.p_floor;
+  memcpy(h.gravity, d.gravity, sizeof h.gravity);
   size_t off[GK_G_COUNT + 5];
   const size_t need = blob_layout(&h, off);
   if (buf && cap >= need) {
@@ -408,7 +412,7 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   *out = NULL;
   gk_blob_head h;
   memcpy(&h, blob, sizeof h);
-  if (h.magic != GK_BLOB_MAGIC || h.version != 2) return GK_ERR_ARG;
+  if (h.magic != GK_BLOB_MAGIC || h.version != 3) return GK_ERR_ARG;
   size_t off[GK_G_COUNT + 5];
   if (blob_layout(&h, off) > n) return GK_ERR_ARG;
   const char *b = (const char *)blob;
@@ -451,6 +455,9 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   memcpy(d.b0_face, h.b0_face, sizeof d.b0_face);
   memcpy(d.b0_edge, h.b0_edge, sizeof d.b0_edge);
   memcpy(d.b0_lorentz, h.b0_lorentz, sizeof d.b0_lorentz);
+  d.rho_floor = h.rho_floor;
+  d.p_floor = h.p_floor;
+  memcpy(d.gravity, h.gravity, sizeof d.gravity);
   d.ring = h.ring;
   memcpy(d.ring_pole, h.ring_pole, sizeof d.ring_pole);
   d.ring_rows = h.ring_rows;
@@ -581,48 +588,72 @@ int solve(int nt) {
     io->extra(BASE_EXT_NAME);
     last_dump_time = 0;
   }
-  const int per_step = dumps || (time_stop < 1e300 && time_stop > 0.0);
-  if (!per_step) {
-    rc = gk_step(g_dev, nt);
-    if (rc != GK_OK) {
-      log_error("Error: device step failed (%d).", rc);
+  /* The loop of src/solver_solve.c:102-529 in runs of steps: the device
+     ends a run after the first step at which a dump or status line is due
+     or time_stop is passed (gk_step_until), so the host meets the checks of
+     every step that could trigger one, and no other step costs a round
+     trip.  Only the last step of a run can meet them. */
+  double sc[6];
+  if ((rc = gk_get_scalars(g_dev, sc)) != GK_OK) {
+    log_error("Error: device scalars failed (%d).", rc);
+    return -1;
+  }
+  int n = 0;
+  while (n < nt) {
+    const double t_dump = dumps ? output_interval * (hdf_seq_num + 1) : INFINITY;
+    const double t_log = 0.1 * output_interval * log_seq_num;
+    const double t_event = fmin(t_dump, t_log);
+    /* steps requested: all that are left, or about what reaching the next
+       stop takes at the next step's dt (with a margin), so that a stop far
+       short of nt does not leave many launched steps with nothing to do;
+       a run that falls short just continues in the next pass */
+    int want = nt - n;
+    const double t_next = fmin(t_event, time_stop);
+    const double dt_next = sc[4];
+    if (t_next < INFINITY && dt_next > 0.0 && dt_next < INFINITY) {
+      const double est = (t_next - sc[2]) / dt_next;
+      if (est < (double)want) want = (int)fmax(1.0, ceil(1.25 * est) + 2.0);
+      if (want > nt - n) want = nt - n;
+    }
+    int done = 0;
+    rc = gk_step_until(g_dev, want, time_stop, t_event, &done);
+    const int bad = rc == GK_ERR_NAN;
+    if (rc != GK_OK && !bad) {
+      log_error("Error: device step %d failed (%d).", n + 1, rc);
       return -1;
     }
-  } else {
-    for (int n = 1; n <= nt; n++) {
-      rc = gk_step(g_dev, 1);
-      if (rc != GK_OK) {
-        log_error("Error: device step %d failed (%d).", n, rc);
-        return -1;
-      }
-      double sc[6];
-      if ((rc = gk_get_scalars(g_dev, sc)) != GK_OK) {
-        log_error("Error: device scalars failed (%d).", rc);
-        return -1;
-      }
-      if (sc[5] != 0.0) log_error("non-finite state detected at step %d", n);
-      time_sim = sc[2];
-      dt = sc[0];
-      divB_max = sc[3];
-      if (dumps && time_sim >= output_interval * (hdf_seq_num + 1)) {
-        if ((rc = gkh_device_fetch()) != GK_OK) {
-          log_error("Error: device state download failed (%d).", rc);
-          return -1;
-        }
-        io->dump(BASE_DAT_NAME);
-        log_info("Output data at time %.15f", time_sim);
-        last_dump_time = time_sim;
-      }
-      if (time_sim >= 0.1 * output_interval * log_seq_num) {
-        log_info("Status at step %d: rank 0, Time %.15f, dt %.15f, max_divB for this rank is %.4e", n,
-                 time_sim, dt, divB_max);
-        log_seq_num++;
-      }
-      if (time_sim > time_stop) {
-        log_info("Step %d, Time %f, dt %f", n, time_sim, dt);
-        break;
-      }
+    n += done;
+    if ((rc = gk_get_scalars(g_dev, sc)) != GK_OK) {
+      log_error("Error: device scalars failed (%d).", rc);
+      return -1;
     }
+    time_sim = sc[2];
+    dt = sc[0];
+    divB_max = sc[3];
+    if (bad) {
+      log_error("Error: non-finite state after step %d (time %.15f)", n, time_sim);
+      return -1;
+    }
+    if (dumps && time_sim >= output_interval * (hdf_seq_num + 1)) {
+      if ((rc = gkh_device_fetch()) != GK_OK) {
+        log_error("Error: device state download failed (%d).", rc);
+        return -1;
+      }
+      io->dump(BASE_DAT_NAME);
+      log_info("Output data at time %.15f", time_sim);
+      last_dump_time = time_sim;
+    }
+    if (time_sim >= 0.1 * output_interval * log_seq_num) {
+      log_info("Status at step %d: rank 0, Time %.15f, dt %.15f, max_divB for this rank is %.4e", n,
+               time_sim, dt, divB_max);
+      log_seq_num++;
+    }
+    if (time_sim > time_stop) {
+      log_info("Step %d, Time %f, dt %f", n, time_sim, dt);
+      log_info("This is the end");
+      break;
+    }
+    if (done == 0) break; /* nothing left to step */
   }
   rc = gkh_device_fetch();
   if (rc != GK_OK) {

@@ -247,6 +247,17 @@ int gk_fill_desc(gk_grid_desc *d, const gk_tables *t, int precision) {
   memcpy(d->b0_cell, gk_b0.cell, sizeof d->b0_cell);
   memcpy(d->b0_face, gk_b0.face, sizeof d->b0_face);
   memcpy(d->b0_edge, gk_b0.edge, sizeof d->b0_edge);
+  /* the problem's floors and gravity (globals a plugin may set) */
+  d->rho_floor = rho_floor;
+  d->p_floor = p_floor;
+  d->gravity[0] = G01;
+  d->gravity[1] = G02;
+  d->gravity[2] = G03;
+  if (!(rho_floor > 0.0) || !(p_floor > 0.0)) {
+    log_error("device solver: floors must be positive (rho_floor %g, p_floor %g)", rho_floor,
+              p_floor);
+    return GK_ERR_ARG;
+  }
   if (doRingAverage == 1) {
     d->ring = 1;
     d->ring_rows = gk_ring_table(config.nk_global, d->ring_nchunk);

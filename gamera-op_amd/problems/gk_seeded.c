@@ -46,6 +46,23 @@
  *                      boundary at x = 0 (rho = 1, v = 0, p = 1, face B = 0
  *                      in the ghosts), extrapolated at x = 1, symmetric in y,
  *                      periodic in z.
+ *   GK_ENTROPY_BOX     GK_SINUSOID_SPHERE with the entropy channel carried:
+ *                      p_S = p at the start (every other seeded problem has
+ *                      p_S = 0).
+ *   GK_FLOOR_BOX       Cartesian periodic box whose state crosses the
+ *                      positivity floors: rho = 1 + 0.05 xi, p = 0.5 outside
+ *                      a seeded sphere (R = 0.2) that holds rho = 1.5 x the
+ *                      density floor and p = 1.02 x the pressure floor and
+ *                      expands uniformly (v = 2.5 d/R radially, d the distance
+ *                      from its centre), so pressure and then density fall
+ *                      below the floors within a few steps (at 24 x 20 x 12
+ *                      cells both branches of the positivity check are
+ *                      taken, the pressure-only one first); the sinusoid's magnetic
+ *                      field.  The plugin sets non-default floors
+ *                      (rho_floor = 0.01, p_floor = 0.002) and a uniform
+ *                      gravity of 0.4 in a seeded direction (G01..G03), the
+ *                      globals the reference leaves to problem files
+ *                      (src/config.c:89-97).
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -73,6 +90,8 @@ enum {
   GK_POLE_CAP = 5,
   GK_WALL_BOX = 6,
   GK_INFLOW_BOX = 7,
+  GK_ENTROPY_BOX = 8,
+  GK_FLOOR_BOX = 9,
 };
 
 
@@ -192,6 +211,21 @@ void set_problem_config() {
   gk_draw_parameters();
   CA = 10.0;
   doRingAverage = gk_polar() ? 1 : 0;
+  /* floors and gravity: the reference's initial values unless the floor box
+     sets its own (every kind sets them, so one process can set up several
+     problems in turn) */
+  rho_floor = 0.00316227;
+  p_floor = 0.000316227;
+  G01 = G02 = G03 = 0.0;
+  if (gk_problem_kind == GK_FLOOR_BOX) {
+    const double cz = 2.0 * gk_scalar(20) - 1.0, az = 2.0 * PI * gk_scalar(21);
+    const double sz = sqrt(fmax(0.0, 1.0 - cz * cz));
+    rho_floor = 0.01;
+    p_floor = 0.002;
+    G01 = 0.4 * sz * cos(az);
+    G02 = 0.4 * sz * sin(az);
+    G03 = 0.4 * cz;
+  }
   if (gk_problem_kind == GK_DIPOLE_SHELL || gk_polar()) {
     x1min_global = 1.0;
     x1max_global = 10.0;
@@ -384,6 +418,35 @@ static void gk_init_pole_sphere(int whole_box) {
          Field_Default, Field_Default);
 }
 
+/* the floor box: an expanding low-density, low-pressure sphere */
+static void gk_init_floor_box(void) {
+  for (int s = 0; s < NS1; s++) {
+    for (int i = is; i <= ie; i++) {
+      for (int j = js; j <= je; j++) {
+        for (int k = ks; k <= ke; k++) {
+          int64_t gi = i - is + proc_coords[0] * config.ni;
+          int64_t gj = j - js + proc_coords[1] * config.nj;
+          int64_t gk = k - ks + proc_coords[2] * config.nk;
+          double xi = 2.0 * gk_hash_u01(gk_problem_seed, 100, gi, gj, gk) - 1.0;
+          double x = x1c[i][j][k], y = x2c[i][j][k], z = x3c[i][j][k];
+          double d[3] = {x - p_ctr[0], y - p_ctr[1], z - p_ctr[2]};
+          for (int n = 0; n < 3; n++) d[n] -= floor(d[n] + 0.5);
+          double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          double sm = gk_ramp(r, 0.2, dx1[i][j][k]);
+          double rho = 1.0 + 0.05 * xi;
+          gas[s][gas_rho][i][j][k] = rho + (1.5 * rho_floor - rho) * sm;
+          gas[s][gas_p][i][j][k] = 0.5 + (1.02 * p_floor - 0.5) * sm;
+          for (int n = 0; n < 3; n++)
+            gas[s][gas_v1 + n][i][j][k] = 2.5 * d[n] / 0.2 * sm;
+        }
+      }
+    }
+  }
+  A3 = gk_Az_sinusoid;
+  AinitB(gem[mag_bi], gem[mag_bj], gem[mag_bk], A1, A2, A3, Field_Default,
+         Field_Default, Field_Default);
+}
+
 void problem_init() {
   log_info("Initializing seeded problem kind %d seed %llu", gk_problem_kind,
            (unsigned long long)gk_problem_seed);
@@ -398,6 +461,17 @@ void problem_init() {
     case GK_WALL_BOX:
     case GK_INFLOW_BOX:
       gk_init_sinusoid(1);
+      break;
+    case GK_ENTROPY_BOX:
+      gk_init_sinusoid(1);
+      for (int s = 0; s < NS1; s++)
+        for (int i = is; i <= ie; i++)
+          for (int j = js; j <= je; j++)
+            for (int k = ks; k <= ke; k++)
+              gas[s][gas_p_S][i][j][k] = gas[s][gas_p][i][j][k];
+      break;
+    case GK_FLOOR_BOX:
+      gk_init_floor_box();
       break;
     case GK_DIPOLE_SHELL:
       gk_init_dipole_shell();

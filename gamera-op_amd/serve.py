@@ -5,9 +5,13 @@ arrive over time.  They are admitted while the device-memory budget and the
 slot count allow, their state stays on the device, and every admitted solve
 advances in the same batched steps (one launch per stage for all of them):
 continuous batching, where requests join and leave the batch between step
-quanta.  The host set-up of newly admitted requests runs while the GPU
-executes the current quantum.  A solve's results do not depend on what else
-is in the batch (batched stepping runs each grid's own arithmetic).
+quanta.  Each member carries its own step budget on the device, so a quantum
+has a fixed length: a member whose steps run out inside it stops there while
+the others go on, and leaves at the end of the quantum.  The host set-up of
+newly admitted requests runs while the GPU executes the current quantum.  A
+solve's results do not depend on what else is in the batch (batched stepping
+runs each grid's own arithmetic).  A request whose state turns non-finite is
+evicted at the end of the quantum in which it does, alone.
 
 The plain loop (`serve_plain`) is the batching-off baseline: one request at a
 time, set up, stepped to completion and retired before the next starts.
@@ -38,6 +42,7 @@ class Request:
     shape: tuple
     steps: int
     precision: int       # 0 FP64, 1 FP32-bulk
+    poison: bool = False  # test hook: one initial density is NaN
 
     @property
     def cells(self):
@@ -54,6 +59,7 @@ class Result:
     cell_updates: int
     time_sim: float
     digest: str = ""
+    status: str = "ok"   # "ok", or "nonfinite" (evicted)
     extra: dict = field(default_factory=dict)
 
     @property
@@ -92,12 +98,20 @@ def device_bytes_estimate(shape, precision):
     return 8 * fs * (12 + 9 + 9) + esz * fs * (27 + 6 + 3) + copies
 
 
-def state_digest(solver):
-    prim, bf = solver.state()
+def state_digest_of(prim, bf):
+    """SHA-256 of a state: 6 primitive and 3 face-B arrays (reference layout)."""
     h = hashlib.sha256()
-    for a in prim + bf:
+    for a in list(prim) + list(bf):
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def state_digest(solver):
+    prim, bf = solver.state()
+    return state_digest_of(prim, bf)
+
+
+POISON_CELL = (4, 4, 4)  # first active cell
 
 
 class _Setup:
@@ -118,7 +132,11 @@ class _Setup:
                 self.P.close()
             self.P = go.Problem(r.kind, r.seed, *r.shape)
             self.key = key
+        if r.poison:
+            self.P.gas(0)[POISON_CELL] = np.nan
         S = go.Solver(self.P, r.precision)
+        if r.poison:
+            self.key = None  # the host state no longer matches the seed
         if tune:
             S.autotune(2)
         return S
@@ -188,13 +206,15 @@ class Preparer:
         return self.pool.apply_async(_worker_prepare, (r.kind, r.seed, r.shape))
 
     @staticmethod
-    def solver(result, precision):
+    def solver(result, precision, poison=False):
         name, blob, shape = result
         shm = shared_memory.SharedMemory(name=name)
         try:
             nb = int(np.prod(shape)) * 8
             arrs = [np.ndarray(shape, dtype=np.float64, buffer=shm.buf, offset=q * nb)
                     for q in range(9)]
+            if poison:
+                arrs[0][POISON_CELL] = np.nan
             S = go.Solver.from_parts(blob, arrs[:6], arrs[6:], precision)
             del arrs
         finally:
@@ -207,11 +227,35 @@ class Preparer:
         self.pool.join()
 
 
+def tune_shapes(requests, trial_steps=2):
+    """Autotune every (problem kind, grid, precision) of a request mix once,
+    before serving: the picks are cached in the library per shape, so the
+    serving loop only applies them and never times trial steps while a
+    quantum runs (contended timings would be noise).  Returns seconds spent."""
+    t0 = time.perf_counter()
+    seen = set()
+    for r in requests:
+        key = (r.kind, tuple(r.shape), r.precision)
+        if key in seen:
+            continue
+        seen.add(key)
+        P = go.Problem(r.kind, r.seed, *r.shape)
+        try:
+            S = go.Solver(P, r.precision)
+            S.autotune(trial_steps)
+            S.close()
+        finally:
+            P.close()
+    return time.perf_counter() - t0
+
+
 def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, digests=False,
           realtime=True, log=None, preparer=None, lookahead=None):
     """Run the request stream through the continuous-batching loop.  With a
     Preparer, request set-up runs in its worker processes (up to `lookahead`
     requests ahead); otherwise in this process, one request per quantum.
+    With tune, admitted grids take the launch shapes cached for their shape
+    (tune_shapes fills the cache; a shape without a pick keeps the defaults).
     Returns (results, wall seconds)."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     setup = _Setup() if preparer is None else None
@@ -221,6 +265,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     preparing = collections.deque()  # (request, async result) in arrival order
     active = []  # [request, solver, steps_left, admitted_time, bytes]
     pending = []
+    leaving = []  # [member, status]: out of the batch, results still to read
     results = []
     used = 0
     nxt = 0
@@ -234,13 +279,32 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     def admit(r, S):
         nonlocal used
         if tune:
-            S.autotune(2)  # measured once per grid shape, then cached in the library
+            S.autotune(0)  # the cached pick for this shape, if any; times nothing
         b = S.device_bytes()
         used += b
-        pending.append([r, S, r.steps, now(), b])
+        a = [r, S, r.steps, now(), b]
+        if r.steps <= 0:  # nothing to step: done at admission
+            leaving.append((a, "ok"))
+        else:
+            pending.append(a)
+
+    def retire(a, status):
+        nonlocal used
+        r, S = a[0], a[1]
+        sc = S.scalars()
+        done_steps = r.steps - max(a[2], 0)
+        res = Result(r.rid, r.arrival, a[3], now(), done_steps, r.cells * done_steps,
+                     sc["time"], status=status)
+        if digests:
+            res.digest = state_digest(S)
+        results.append(res)
+        S.close()
+        used -= a[4]
+        if log:
+            log(f"{status} rid={r.rid} latency={res.latency:.3f}s active={len(active)}")
 
     try:
-        while nxt < len(reqs) or queue or preparing or active or pending:
+        while nxt < len(reqs) or queue or preparing or active or pending or leaving:
             tn = now()
             while nxt < len(reqs) and (reqs[nxt].arrival <= tn or not realtime):
                 queue.append(reqs[nxt])
@@ -259,7 +323,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                         break
                     _, res = preparing.popleft()
                     ta = time.perf_counter()
-                    admit(r, Preparer.solver(res.get(), r.precision))
+                    admit(r, Preparer.solver(res.get(), r.precision, r.poison))
                     stats["admit_s"] += time.perf_counter() - ta
             else:
                 # host set-up in this process: one request per pass while the
@@ -277,40 +341,39 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                         break
             if in_flight:
                 ta = time.perf_counter()
-                B.sync()
+                bad = {id(S) for S in B.sync_status()}
                 stats["gpu_wait_s"] += time.perf_counter() - ta
-                done = []
-                for a in active:
-                    a[2] -= in_flight
-                    if a[2] == 0:
-                        done.append(a)
                 in_flight = 0
-                for a in done:
-                    r, S = a[0], a[1]
-                    B.remove(S)
-                    sc = S.scalars()
-                    res = Result(r.rid, r.arrival, a[3], now(), r.steps, r.cells * r.steps,
-                                 sc["time"])
-                    if digests:
-                        res.digest = state_digest(S)
-                    results.append(res)
-                    S.close()
-                    used -= a[4]
-                    active.remove(a)
-                    if log:
-                        log(f"done rid={r.rid} latency={res.latency:.3f}s active={len(active)}")
+                # members whose budget ran out, or whose state turned
+                # non-finite, leave the batch before the next quantum
+                for a in list(active):
+                    a[2] = B.budget(a[1])
+                    status = "nonfinite" if id(a[1]) in bad else ("ok" if a[2] == 0 else None)
+                    if status:
+                        B.remove(a[1])
+                        active.remove(a)
+                        leaving.append((a, status))
             for p in pending:
                 B.add(p[1])
+                B.set_budget(p[1], p[2])
                 active.append(p)
             pending = []
             if active:
-                n = min(quantum, min(a[2] for a in active))
                 ta = time.perf_counter()
-                B.step_async(n)
+                B.step_async(quantum)
                 stats["launch_s"] += time.perf_counter() - ta
                 stats["quanta"] += 1
-                in_flight = n
-            elif preparing:
+                in_flight = quantum
+            # read back the requests that left while the next quantum runs
+            ta = time.perf_counter()
+            for a, status in leaving:
+                retire(a, status)
+            if leaving:
+                stats["retire_s"] += time.perf_counter() - ta
+            leaving = []
+            if in_flight:
+                continue
+            if preparing:
                 ta = time.perf_counter()
                 preparing[0][1].wait(0.05)
                 stats["prep_wait_s"] += time.perf_counter() - ta
@@ -335,8 +398,9 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     return results, wall
 
 
-def serve_plain(requests, tune=False, digests=False, realtime=True, log=None):
-    """Batching off: one request at a time, in arrival order."""
+def serve_plain(requests, tune=True, digests=False, realtime=True, log=None):
+    """Batching off: one request at a time, in arrival order.  With tune, each
+    grid takes the launch shape cached for its shape (timed now if none is)."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     setup = _Setup()
     results = []
@@ -349,10 +413,14 @@ def serve_plain(requests, tune=False, digests=False, realtime=True, log=None):
                     time.sleep(wait)
             adm = time.perf_counter() - t0
             S = setup.solver(r, tune)
-            S.step(r.steps)
+            status = "ok"
+            try:
+                S.step(max(r.steps, 0))
+            except go.GkNonFinite:
+                status = "nonfinite"
             sc = S.scalars()
-            res = Result(r.rid, r.arrival, adm, time.perf_counter() - t0, r.steps,
-                         r.cells * r.steps, sc["time"])
+            res = Result(r.rid, r.arrival, adm, time.perf_counter() - t0, max(r.steps, 0),
+                         r.cells * max(r.steps, 0), sc["time"], status=status)
             if digests:
                 res.digest = state_digest(S)
             results.append(res)
@@ -368,6 +436,7 @@ def summarize(results, wall):
     lat = np.array([r.latency for r in results])
     cu = sum(r.cell_updates for r in results)
     return {"requests": len(results), "wall_s": wall, "cell_updates": cu,
+            "nonfinite": sum(1 for r in results if r.status != "ok"),
             "cell_updates_per_s": cu / wall if wall > 0 else 0.0,
             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
@@ -407,11 +476,13 @@ def main(argv=None):
     if not args.no_serve:
         preparer = Preparer(args.workers or None)  # before this process touches the GPU
         try:
+            tuning = tune_shapes(reqs)
             res, wall = serve(reqs, budget_bytes=args.budget_gib << 30, preparer=preparer,
                               log=progress)
         finally:
             preparer.close()
         out["served"] = summarize(res, wall)
+        out["served"]["tuning_s_before_stream"] = round(tuning, 3)
         out["served"]["loop_seconds"] = {k: round(v, 3) if isinstance(v, float) else v
                                          for k, v in serve.last_stats.items()}
     if args.plain or args.no_serve:

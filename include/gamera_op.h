@@ -107,6 +107,16 @@ typedef struct {
   int ring_pole[2];
   int ring_rows;
   int ring_nchunk[16];
+  /* positivity floors and uniform gravity: the reference's mutable globals
+     rho_floor, p_floor (src/config.c:89-90) and G01..G03 (:95-97), which a
+     problem file may set; applied in check_positivity
+     (src/solver_func.c:2117-2135) and the momentum update
+     (src/solver_solve.c:469-471).  A floor of 0 selects the reference's
+     initial value (0.00316227, 0.000316227), so a zero-initialised
+     descriptor is the reference's default set-up; negative or non-finite
+     values are rejected (GK_ERR_ARG). */
+  double rho_floor, p_floor;
+  double gravity[3];
 } gk_grid_desc;
 
 typedef struct gk_solver gk_solver;
@@ -127,8 +137,18 @@ int gk_set_state(gk_solver *s, const double *const prim[6], const double *const 
    value passed in (restart). */
 int gk_prologue(gk_solver *s, double dt0_restart);
 
-/* Advance n AB2 steps; dt, dt0 and time stay on the device. */
+/* Advance n AB2 steps; dt, dt0 and time stay on the device.  Returns
+   GK_ERR_NAN when the state holds a non-finite value after the steps (the
+   cell update's fused check; the steps are taken regardless). */
 int gk_step(gk_solver *s, int n);
+
+/* Up to n steps of the reference's solve() loop (src/solver_solve.c:102-529)
+   without a host round trip per step: the device ends the run after the
+   first step whose time exceeds t_stop (the loop's time_stop break,
+   :523-527) or reaches t_event (time >= t_event: the caller's next dump or
+   status line is due, :510-521).  *done (may be NULL) receives the steps
+   taken.  Returns GK_ERR_NAN as gk_step does. */
+int gk_step_until(gk_solver *s, int n, double t_stop, double t_event, int *done);
 
 /* Download state and history (any pointer may be NULL). */
 int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], double *prim_prev[6],
@@ -162,7 +182,8 @@ size_t gk_device_bytes(const gk_solver *s);
    that on grids large enough to fill the GPU with it); GK_TUNE_EMF_CHUNK:
    about how many edges each thread walks in the edge-field kernels
    (1..4096, default 16).  A column is split into near-equal chunks of
-   about that length (within 1.5x of it). */
+   about that length (within 1.5x of it).  GK_ERR_STATE while the solver is
+   in a batch (the batch's launch tables hold its shapes). */
 enum { GK_TUNE_CHUNK = 0, GK_TUNE_EMF_CHUNK = 1 };
 int gk_set_tuning(gk_solver *s, int param, int value);
 int gk_get_tuning(const gk_solver *s, int param);
@@ -171,7 +192,8 @@ int gk_get_tuning(const gk_solver *s, int param);
    trial_steps steps for each candidate march length on this grid, keeps the
    fastest (written to *best) and restores the state it started from, so
    results do not depend on tuning.  Picks are cached per grid shape and
-   precision for the life of the process. */
+   precision for the life of the process.  trial_steps = 0 only applies a
+   cached pick (*best = 0 when there is none) and times nothing. */
 int gk_autotune(gk_solver *s, int trial_steps, int *best);
 
 /* Batched stepping (no counterpart in the reference; for ensembles and the
@@ -189,7 +211,20 @@ int gk_batch_remove(gk_batch *b, gk_solver *s);
 int gk_batch_size(const gk_batch *b);
 int gk_batch_step(gk_batch *b, int n);        /* n steps, waits for them */
 int gk_batch_step_async(gk_batch *b, int n);  /* n steps, returns at once */
+/* waits for the launched steps; GK_ERR_NAN if a member's state holds a
+   non-finite value (gk_batch_status names which) */
 int gk_batch_sync(gk_batch *b);
+/* Step budget of a member (serving): it takes `steps` more steps, counted
+   from the steps already launched, and then stops on the device while the
+   batch goes on (its later launches do nothing), so one batch step count
+   need not stop at the member that finishes first.  steps < 0: no limit
+   (the default on gk_batch_add).  gk_batch_budget returns what is left of
+   it after the launched steps (-1: no limit). */
+int gk_batch_set_budget(gk_batch *b, gk_solver *s, long long steps);
+long long gk_batch_budget(gk_batch *b, gk_solver *s);
+/* after gk_batch_sync: GK_ERR_NAN if this member's state holds a
+   non-finite value, GK_OK if not, GK_ERR_ARG if it is not a member */
+int gk_batch_status(gk_batch *b, gk_solver *s);
 int gk_batch_set_graphs(gk_batch *b, int on); /* default on */
 int gk_batch_enable_timing(gk_batch *b, int on);
 int gk_batch_kernel_times(gk_batch *b, const char **names, double *ms, int *n);

@@ -66,9 +66,10 @@ enum {
 static const double gamma_val = 5.0 / 3.0;
 static const double CFL = 0.3;
 static const double PDMB = 1.0;
-static const double rho_floor = 0.00316227;
-static const double p_floor = 0.000316227;
-static const double G0[3] = {0.0, 0.0, 0.0};
+/* floors and gravity are mutable globals a problem plugin may set
+   (src/config.c:89-97); these are their initial values */
+static const double rho_floor_default = 0.00316227;
+static const double p_floor_default = 0.000316227;
 
 struct go_solver {
   int ni, nj, nk, NI, NJ, NK;
@@ -81,6 +82,7 @@ struct go_solver {
   double *crd[12]; /* x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
   double dt, dt0, time_sim, divB_max;
   double *dbgF; /* optional per-direction copies of the 9 face fluxes */
+  double rho_floor, p_floor, G0[3];
   int ring, pole[2], navr, nchunk[16]; /* pole ring averaging */
 };
 
@@ -113,6 +115,8 @@ go_solver *go_create(int ni, int nj, int nk, int coord, const int bc[6],
   s->coord = coord;
   for (int n = 0; n < 6; n++) s->bc[n] = bc[n];
   s->CA = CA;
+  s->rho_floor = rho_floor_default;
+  s->p_floor = p_floor_default;
   s->n3 = (int64_t)s->NI * s->NJ * s->NK;
   s->gas = (double *)calloc((size_t)(NF_gas * s->n3), sizeof(double));
   s->gem = (double *)calloc((size_t)(NF_gem * s->n3), sizeof(double));
@@ -156,6 +160,9 @@ double go_scalar(const go_solver *s, int which) {
     case 0: return s->dt;
     case 1: return s->dt0;
     case 2: return s->time_sim;
+    case 4: return s->rho_floor;
+    case 5: return s->p_floor;
+    case 6: case 7: case 8: return s->G0[which - 6];
     default: return s->divB_max;
   }
 }
@@ -165,6 +172,9 @@ void go_set_scalar(go_solver *s, int which, double v) {
     case 0: s->dt = v; break;
     case 1: s->dt0 = v; break;
     case 2: s->time_sim = v; break;
+    case 4: s->rho_floor = v; break;
+    case 5: s->p_floor = v; break;
+    case 6: case 7: case 8: s->G0[which - 6] = v; break;
     default: s->divB_max = v; break;
   }
 }
@@ -1178,11 +1188,11 @@ static void step_once(go_solver *s) {
         GAS(gas_v3, i, j, k) = v3;
         GAS(gas_p, i, j, k) = (GAS(gasc_eng, i, j, k) - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) * rho) * (gamma_val - 1.0);
         GAS(gas_p_S, i, j, k) = GAS(gasc_entropy, i, j, k) * pow(rho, gamma_val - 1.0);
-        if (GAS(gas_rho, i, j, k) < rho_floor) {
-          GAS(gas_rho, i, j, k) = rho_floor;
-          GAS(gas_p, i, j, k) = p_floor;
+        if (GAS(gas_rho, i, j, k) < s->rho_floor) {
+          GAS(gas_rho, i, j, k) = s->rho_floor;
+          GAS(gas_p, i, j, k) = s->p_floor;
         }
-        if (GAS(gas_p, i, j, k) < p_floor) GAS(gas_p, i, j, k) = p_floor;
+        if (GAS(gas_p, i, j, k) < s->p_floor) GAS(gas_p, i, j, k) = s->p_floor;
 
         GEM(mags_b1, i, j, k) = GEM(mag_bi, i, j, k) * (X1(i + 1, j, k) - X1CTR(i, j, k)) / DX1(i, j, k)
             + GEM(mag_bi, i + 1, j, k) * (X1CTR(i, j, k) - X1(i, j, k)) / DX1(i, j, k);
@@ -1215,7 +1225,7 @@ static void step_once(go_solver *s) {
                       perp * ((-dt * GAS(dgascf_rhov1 + m, i, j, k) + dt * GAS(S1_HD + m, i, j, k)) +
                               (-dt * GEM(dmagsf_x1dir + m, i, j, k) + dt * GEM(S1_B + m, i, j, k)) +
                               dva * v0[m] + mt[m] * bdv);
-          rv += dt * GAS(gasc_rho, i, j, k) * G0[m];
+          rv += dt * GAS(gasc_rho, i, j, k) * s->G0[m];
           GAS(gasc_rhov1 + m, i, j, k) = rv;
         }
         for (int m = 0; m < 3; m++)

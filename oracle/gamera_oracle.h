@@ -83,7 +83,9 @@ void go_face_flux(int dir, const double L[6], const double R[6], const double bl
 void go_ring_filter(double *Q, int N, double *Q0);
 void go_ring_block_avg(double *Q, int N, int nch);
 
-/* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max */
+/* scalars: 0 dt, 1 dt0, 2 time_sim, 3 divB_max, 4 rho_floor, 5 p_floor,
+   6..8 gravity G01..G03 (src/config.c:89-97; settable like the reference's
+   globals) */
 double go_scalar(const go_solver *s, int which);
 void go_set_scalar(go_solver *s, int which, double v);
 

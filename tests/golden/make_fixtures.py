@@ -41,6 +41,10 @@ STATE_CASES = [
     (6, 15, 16, 12, 8, 12),
     # fixed boundary at x = 0, extrapolated at x = 1, symmetric in y
     (7, 22, 16, 12, 8, 12),
+    # entropy channel carried (p_S = p at the start)
+    (8, 32, 16, 16, 8, 10),
+    # positivity floors crossed (both branches), non-default floors, gravity
+    (9, 31, 24, 20, 12, 20),
 ]
 GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
               (4, 8, 12, 16, 32)]

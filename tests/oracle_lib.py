@@ -110,11 +110,14 @@ class Oracle:
     def step(self, n=1):
         self.lib.go_step(self.h, n)
 
+    SCALARS = ["dt", "dt0", "time_sim", "divB_max", "rho_floor", "p_floor", "G01", "G02", "G03"]
+
     def scalar(self, name):
-        return self.lib.go_scalar(self.h, ["dt", "dt0", "time_sim", "divB_max"].index(name))
+        return self.lib.go_scalar(self.h, self.SCALARS.index(name))
 
     def set_scalar(self, name, v):
-        self.lib.go_set_scalar(self.h, ["dt", "dt0", "time_sim", "divB_max"].index(name), v)
+        q = name if isinstance(name, int) else self.SCALARS.index(name)
+        self.lib.go_set_scalar(self.h, q, v)
 
     def close(self):
         if self.h:

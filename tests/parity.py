@@ -37,6 +37,11 @@ def oracle_from_problem(P, fast=False):
         O.gas[f] = P.gas(f)
     for f in range(3):
         O.gem[f] = P.gem(f)
+    # the problem's floors and gravity (globals the plugin sets)
+    import ctypes
+    h = P.h
+    for q, name in ((4, "rho_floor"), (5, "p_floor"), (6, "G01"), (7, "G02"), (8, "G03")):
+        O.set_scalar(q, ctypes.c_double.in_dll(h, name).value)
     if P.kind == 4:  # full sphere: ring averaging at both poles
         O.set_ring(1, 1, 1)
     if P.kind == 5:  # northern hemisphere: the pole at theta = 0 only

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|void|double)\s+\**(gk_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|void|double|long long)\s+\**(gk_\w+)\s*\(", text, re.M)))
 
 
 def test_device_library_exports_header_symbols():

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import gamera_op as go
+from parity import exact_mismatches, run_oracle
 
 MIX = [
     # kind, seed, ni, nj, nk, precision
@@ -66,6 +67,25 @@ def test_batch_equals_alone(graphs):
             print(q, "time", sa["time"], sb["time"], flush=True)
             assert sa["time"] == sb["time"] and sa["dt"] == sb["dt"]
             assert same(alone, inb), q
+        if graphs:
+            # every FP64 grid of the mix is bitwise the strict oracle's after
+            # the same 18 steps (the batch joined and left around it)
+            for q, (kind, seed, ni, nj, nk, prec) in enumerate(MIX):
+                if prec != 0:
+                    continue
+                P = go.Problem(kind, seed, ni, nj, nk)
+                try:
+                    O = run_oracle(P, 18)
+                    prim, bf = pairs[q][1].state()
+                    for f in range(6):
+                        P.gas(f)[...] = prim[f]
+                    for f in range(3):
+                        P.gem(f)[...] = bf[f]
+                    nbad = exact_mismatches(P, O)
+                    print(q, "vs oracle: bitwise mismatches", nbad, flush=True)
+                    assert nbad == 0, q
+                finally:
+                    P.close()
     finally:
         B.close()
         for a, b in pairs:
@@ -81,6 +101,15 @@ def test_autotune_keeps_results():
     try:
         best = A.autotune(2)
         print("autotuned chunk", best, flush=True)
+        # a batched solver's launch shapes belong to the batch's tables
+        B = go.Batch()
+        try:
+            B.add(Bs)
+            with pytest.raises(go.GkError):
+                Bs.set_tuning(go.TUNE_CHUNK, 16)
+            B.remove(Bs)
+        finally:
+            B.close()
         assert best in (8, 16, 32, 64, 128, 256)
         assert A.get_tuning(go.TUNE_CHUNK) == best
         A.step(10)

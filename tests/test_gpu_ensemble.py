@@ -1,0 +1,76 @@
+"""The batched-ensemble workload at full shape: 64 Cartesian periodic grids
+drawn by the benchmark's seeded rule from {32^3, 48^3, 64^3, 96^3,
+128x64x32}, half with the over-pressure sphere, FP32-bulk, 300 AB2 steps in
+batched launches under HIP graphs with autotuned launch shapes.
+
+* Every grid equals its own solo run (gk_step) bit for bit.
+* Every grid keeps max |div B| at FP64 roundoff and conserves its total mass
+  (uniform cell volume: the sum of rho) to 1e-12 relative.
+* The first two 32^3 and the first two 48^3 members are within the FP32-bulk
+  bar (relative L1 <= 1e-5 for rho, p, total energy) of the strict FP64
+  oracle after the same 300 steps.  (Two of each, not all of them: the
+  strict oracle takes seconds per 48^3 grid.)
+"""
+import numpy as np
+import pytest
+
+import gamera_op as go
+from ensemble import ensemble_specs
+from parity import active_slices, compare_states, run_oracle
+
+STEPS = 300
+ORACLE_SIZES = {(32, 32, 32): 2, (48, 48, 48): 2}
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_ensemble_64_grids_batched_equals_solo_and_oracle():
+    specs = ensemble_specs(64, 1001)  # the benchmark's default draw
+    print("ensemble sizes", sorted({s[2] for s in specs}), flush=True)
+    batched, solo, mass0 = [], [], []
+    for kind, seed, shape in specs:
+        P = go.Problem(kind, seed, *shape)
+        cell, _ = active_slices(P)
+        mass0.append(float(P.gas(0)[cell].sum()))
+        batched.append(go.Solver(P, 1))
+        solo.append(go.Solver(P, 1))
+        P.close()
+    B = go.Batch()
+    try:
+        for S in batched:
+            S.autotune(2)
+            B.add(S)
+        B.step(STEPS)
+        for S in solo:
+            S.step(STEPS)
+        want = dict(ORACLE_SIZES)
+        for q, ((kind, seed, shape), a, b) in enumerate(zip(specs, batched, solo)):
+            pa, ba = a.state()
+            pb, bb = b.state()
+            assert all(np.array_equal(x, y) for x, y in zip(pa + ba, pb + bb)), (q, shape)
+            sc = a.scalars()
+            assert sc["nonfinite"] == 0.0 and sc["divB"] < 1e-10, (q, sc)
+            n = shape
+            rho = pa[0][4:4 + n[0], 4:4 + n[1], 4:4 + n[2]]
+            assert abs(float(rho.sum()) - mass0[q]) <= 1e-12 * mass0[q], q
+            if want.get(shape, 0) > 0:
+                want[shape] -= 1
+                P = go.Problem(kind, seed, *shape)
+                try:
+                    O = run_oracle(P, STEPS)  # strict IEEE build
+                    for f in range(6):
+                        P.gas(f)[...] = pa[f]
+                    for f in range(3):
+                        P.gem(f)[...] = ba[f]
+                    rel = compare_states(P, O)
+                    print(q, shape, {k: f"{v:.2e}" for k, v in rel.items()}, flush=True)
+                    for k in ("rho", "p", "eng"):
+                        assert rel[k] <= 1e-5, (q, k, rel[k])
+                    assert abs(sc["time"] - O.scalar("time_sim")) <= 1e-5 * O.scalar("time_sim")
+                finally:
+                    P.close()
+        assert not any(want.values()), want
+    finally:
+        B.close()
+        for S in batched + solo:
+            S.close()

@@ -1,4 +1,5 @@
-"""FP32-bulk form vs the FP64 oracle.
+"""FP32-bulk form vs the FP64 oracle (its strict IEEE build, bitwise the
+compiled reference).
 
 Reconstruction, Riemann fluxes, magnetic stress and EMFs run in FP32; state,
 metric factors, the CT face-B update and the dt reduction stay FP64.  Bar:
@@ -39,7 +40,7 @@ def test_fp32_bulk_matches_fp64_oracle(kind, seed, ni, nj, nk, steps):
     print(f"fp32 case kind={kind} seed={seed} {ni}x{nj}x{nk} steps={steps}", flush=True)
     P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
     try:
-        O = run_oracle(P, steps, fast=True)
+        O = run_oracle(P, steps)  # the strict IEEE oracle
         P.device_open(1)
         P.device_step(steps)
         P.device_fetch()

@@ -28,7 +28,11 @@ STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16,
                # reflective walls in x, extrapolated boundaries in y
                (6, 15, 16, 12, 8, 12),
                # fixed boundary at x = 0, extrapolated at x = 1, symmetric y
-               (7, 22, 16, 12, 8, 12)]
+               (7, 22, 16, 12, 8, 12),
+               # entropy channel carried (p_S = p at the start)
+               (8, 32, 16, 16, 8, 10),
+               # positivity floors crossed, non-default floors, gravity
+               (9, 31, 24, 20, 12, 20)]
 
 
 def sha(a):
@@ -53,6 +57,36 @@ def test_oracle_reproduces_reference_bitwise(kind, seed, ni, nj, nk, steps):
         # reduction (src/solver_func.c:324-343), so its value can miss the
         # true maximum but never exceed it
         assert sc[3] <= O.scalar("divB_max") < 1e-12
+    finally:
+        P.close()
+
+
+def test_floor_fixture_pins_floors_and_gravity():
+    """Negative controls: the floor-box fixture takes both branches of the
+    positivity check, and the oracle with the reference's default floors or
+    without gravity does not reproduce it."""
+    kind, seed, ni, nj, nk, steps = 9, 31, 24, 20, 12, 20
+    key = f"k{kind}_s{seed}_{ni}x{nj}x{nk}_n{steps}"
+    P = go.Problem(kind, seed, ni, nj, nk)
+    try:
+        cell, _ = active_slices(P)
+        for change in ({}, {"rho_floor": 0.00316227, "p_floor": 0.000316227},
+                       {"G01": 0.0, "G02": 0.0, "G03": 0.0}):
+            O = oracle_from_problem(P)
+            for name, v in change.items():
+                O.set_scalar(name, v)
+            O.prologue()
+            hit_rho = hit_p = 0
+            for _ in range(steps):
+                O.step(1)
+                rho, p = O.gas[0][cell], O.gas[4][cell]
+                hit_rho += int((rho == 0.01).sum())
+                hit_p += int(((p == 0.002) & (rho > 0.01)).sum())
+            same = all(np.array_equal(O.gas[f][cell], STATES[f"{key}/prim{f}"]) for f in range(6))
+            if not change:
+                assert same and hit_rho > 0 and hit_p > 0, (hit_rho, hit_p)
+            else:
+                assert not same, change
     finally:
         P.close()
 
