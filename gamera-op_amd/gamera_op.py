@@ -211,6 +211,7 @@ class Problem:
         self.n = (sz[0], sz[1], sz[2])
         self.shape = (sz[3], sz[4], sz[5])
         self.kind, self.seed, self.coord = kind, seed, coord
+        self.full_b0 = bool(full_b0)
         self.CA = self.h.gkh_CA()
 
     def _view(self, which, f):

@@ -666,12 +666,19 @@ int solve(int nt) {
 
 /* ------------------------------------------------------------------------ */
 /* Single-GPU stand-ins for the reference's communicator set-up
-   (src/setup_mpi.h:26-29): one rank owns the whole grid, so the periodic
-   images come from the device ghost fill. */
-int setup_comm_and_types(int rank) {
-  (void)rank;
+   (src/setup_mpi.h:7,26-29): one rank owns the whole grid, so the periodic
+   images come from the device ghost fill.  rank and size are the globals a
+   reference main() fills from MPI_COMM_WORLD (src/main.c:27-28); a caller
+   without MPI leaves them at 0 and 1. */
+int rank = 0, size = 1;
+
+int setup_comm_and_types(int rank_) {
   if (config.proc_dims[0] * config.proc_dims[1] * config.proc_dims[2] != 1) {
     log_error("the device solver runs one rank per grid (proc_dims must be 1 1 1)");
+    return -1;
+  }
+  if (rank_ != 0 || size != 1) {
+    log_error("the device solver runs one process per grid (rank %d of %d)", rank_, size);
     return -1;
   }
   config.proc_coords[0] = config.proc_coords[1] = config.proc_coords[2] = 0;

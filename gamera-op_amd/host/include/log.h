@@ -6,8 +6,10 @@
 #ifndef LOG_H
 #define LOG_H
 
+#include <stdarg.h>
 #include <stdbool.h>
 #include <stdio.h>
+#include <time.h> /* the reference header brings it in; src/main.c relies on it */
 
 enum { LOG_TRACE, LOG_DEBUG, LOG_INFO, LOG_WARN, LOG_ERROR, LOG_FATAL };
 

@@ -10,7 +10,9 @@
  * as raw little-endian float64 records so tests can read them with numpy.
  *
  * Usage: ref_driver <config.yaml> <out.bin> <dump_geometry 0|1>
- *   The plugin selection comes from the GK_PROBLEM / GK_SEED environment.
+ *   The seeded plugin takes its problem from the GK_PROBLEM / GK_SEED
+ *   environment; built with a shipped problem file (-DPROBLEM_<name>) the
+ *   driver runs that file instead (oracle/Makefile, target "shipped").
  *   Run it in an empty directory: initialize_solver restarts from any
  *   mhd_*.h5 file it finds in the working directory.
  *
@@ -30,7 +32,9 @@
 #include "setup_mpi.h"
 #include "solver.h"
 
+#ifdef PROBLEM_GK_SEEDED
 int gk_problem_custom_grid(void);
+#endif
 
 static FILE *out;
 
@@ -60,7 +64,9 @@ int main(int argc, char *argv[]) {
   if (setup_comm_and_types(rank) != 0) return 3;
   if (initialize_logging(BASE_LOG_NAME, rank) != 0) return 4;
   if (initialize_solver(BASE_DAT_NAME) != 0) return 5;
+#ifdef PROBLEM_GK_SEEDED
   if (gk_problem_custom_grid()) set_geometry_arrays();
+#endif
   getweights();
   if (!read_restart) problem_init();
   set_background_field();

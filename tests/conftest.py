@@ -9,6 +9,7 @@ for p in (os.path.join(ROOT, "tests"), os.path.join(ROOT, "tests", "golden"),
 
 
 PREPARER = None
+RUNNER = None
 
 
 def pytest_configure(config):
@@ -18,13 +19,19 @@ def pytest_configure(config):
     # never be started from a process that holds a GPU context.
     expr = getattr(config.option, "markexpr", "") or ""
     if "gpu" in expr and "not gpu" not in expr:
-        global PREPARER
+        global PREPARER, RUNNER
         import serve
         PREPARER = serve.Preparer(workers=4)
+        # programs that GPU tests start (reference runs, drop-in builds)
+        import runner
+        RUNNER = runner.Runner()
 
 
 def pytest_unconfigure(config):
-    global PREPARER
+    global PREPARER, RUNNER
     if PREPARER is not None:
         PREPARER.close()
         PREPARER = None
+    if RUNNER is not None:
+        RUNNER.close()
+        RUNNER = None

@@ -73,7 +73,7 @@ def test_batch_equals_alone(graphs):
             for q, (kind, seed, ni, nj, nk, prec) in enumerate(MIX):
                 if prec != 0:
                     continue
-                P = go.Problem(kind, seed, ni, nj, nk)
+                P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
                 try:
                     O = run_oracle(P, 18)
                     prim, bf = pairs[q][1].state()

@@ -55,7 +55,7 @@ def test_ensemble_64_grids_batched_equals_solo_and_oracle():
             assert abs(float(rho.sum()) - mass0[q]) <= 1e-12 * mass0[q], q
             if want.get(shape, 0) > 0:
                 want[shape] -= 1
-                P = go.Problem(kind, seed, *shape)
+                P = go.Problem(kind, seed, *shape, full_b0=True)
                 try:
                     O = run_oracle(P, STEPS)  # strict IEEE build
                     for f in range(6):

@@ -32,7 +32,7 @@ def same_state(a, b):
 @pytest.mark.parametrize("seed,ni,nj,nk,steps", [(31, 24, 20, 12, 20), (33, 32, 24, 16, 25)])
 def test_floor_box_matches_oracle(seed, ni, nj, nk, steps):
     print(f"floor box seed={seed} {ni}x{nj}x{nk} steps={steps}", flush=True)
-    P = go.Problem(go.FLOOR_BOX, seed, ni, nj, nk)
+    P = go.Problem(go.FLOOR_BOX, seed, ni, nj, nk, full_b0=True)
     try:
         O = oracle_from_problem(P)
         O.prologue()
@@ -58,7 +58,7 @@ def test_floor_box_matches_oracle(seed, ni, nj, nk, steps):
 @pytest.mark.gpu
 def test_entropy_channel():
     kind, seed, ni, nj, nk, steps = go.ENTROPY_BOX, 32, 16, 16, 8, 10
-    P = go.Problem(kind, seed, ni, nj, nk)
+    P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
     try:
         O = run_oracle(P, steps)
         P.device_open(0)
@@ -78,7 +78,7 @@ def test_entropy_channel():
     finally:
         P.close()
     # FP32-bulk form: the entropy channel too within the form's bar
-    P = go.Problem(kind, seed, 24, 24, 16)
+    P = go.Problem(kind, seed, 24, 24, 16, full_b0=True)
     try:
         O = run_oracle(P, 20)
         P.device_open(1)
@@ -149,7 +149,7 @@ def test_nonfinite_member_flagged_alone():
 
 @pytest.mark.gpu
 def test_step_until_stops_where_the_reference_loop_breaks():
-    P = go.Problem(go.SINUSOID_BLAST, 7, 16, 16, 16)
+    P = go.Problem(go.SINUSOID_BLAST, 7, 16, 16, 16, full_b0=True)
     try:
         O = oracle_from_problem(P)
         O.prologue()

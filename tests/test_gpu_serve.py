@@ -91,7 +91,7 @@ def test_served_stream_of_the_configured_mix():
     small = [r for r in reqs if r.shape == (32, 32, 32) and r.precision == 0]
     assert small, "the draw has no 32^3 FP64 request"
     for r in small[:2]:
-        P = go.Problem(r.kind, r.seed, *r.shape)
+        P = go.Problem(r.kind, r.seed, *r.shape, full_b0=True)
         try:
             O = run_oracle(P, r.steps)
             S = go.Solver(P, 0)
