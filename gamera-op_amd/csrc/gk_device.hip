@@ -44,6 +44,7 @@
 #include <thread>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "gamera_op.h"
 
@@ -2537,6 +2538,7 @@ struct gk_solver {
   Grid g;
   Geo geo;
   B0c b0;
+  int device;          // HIP device the solver was created on
   int spherical;
   int bc[2][6];
   int ni, nj, nk;
@@ -2673,6 +2675,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   }
 
   gk_solver *s = new gk_solver();
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return GK_ERR_DEVICE;
+  }
   // every upload of the set-up runs on the solver's own stream, so creating
   // a solver does not wait for work other solvers or batches have in flight
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -3729,6 +3735,93 @@ extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], dou
     if (bface && bface[m] && (rc = copy_out(s, bface[m], s->B[s->b_0] + m * fs)) != GK_OK) return rc;
     if (bface_prev && bface_prev[m] &&
         (rc = copy_out(s, bface_prev[m], s->B[s->b_m1] + m * fs)) != GK_OK)
+      return rc;
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return GK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Read-back that allocates nothing (gk_get_state_noalloc).  gk_get_state
+// stages each field through a device staging field and a pinned host buffer
+// that it allocates on first use; this variant stages through `scratch`, a
+// file-scope device array of NOALLOC_SCRATCH_BYTES that every GPU the library
+// runs on holds for the life of the process, and copies from there straight
+// into the caller's arrays.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each GPU comes from
+// hipGetSymbolAddress, looked up once per device (with that device current)
+// and kept in scratch_dev[]; only that address is ever passed to a kernel or
+// a copy.  Every call on one GPU shares the array, so calls that overlap on
+// one GPU (other threads, other streams) race; each field is written in full
+// before it is read, so nothing needs clearing between calls.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;  // one field of up to 4Mi points
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static void *scratch_dev[MAX_DEVICES];
+
+static int scratch_on(int dev, double **out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return GK_ERR_ARG;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "gamera_op: hipGetSymbolAddress(scratch) failed: %s\n",
+              hipGetErrorString(e));
+      return GK_ERR_DEVICE;
+    }
+    scratch_dev[dev] = p;
+  }
+  *out = (double *)scratch_dev[dev];
+  return GK_OK;
+}
+
+// one device field -> the caller's host array (reference layout) via scratch
+static int copy_out_scratch(gk_solver *s, double *st, double *dst, const double *src) {
+  const long rows = (long)s->g.N[0] * s->g.N[1];
+  const int NK = s->g.N[2];
+  k_pitch<<<2048, 256, 0, s->stream>>>(src, st, rows, NK, (int)s->g.st[1], NK);
+  HIPCHK(hipGetLastError());
+  // stream order keeps the next field's re-pitch behind this copy
+  HIPCHK(hipMemcpyAsync(dst, st, (size_t)rows * NK * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  return GK_OK;
+}
+
+extern "C" int gk_get_state_noalloc(gk_solver *s, double *prim[6], double *bface[3],
+                                    double *prim_prev[6], double *bface_prev[3]) {
+  if (!s) return GK_ERR_ARG;
+  const size_t n = (size_t)s->g.N[0] * s->g.N[1] * s->g.N[2];
+  if (n * sizeof(double) > NOALLOC_SCRATCH_BYTES) {
+    fprintf(stderr,
+            "gamera_op: gk_get_state_noalloc: a field of %zu points (%zu bytes) exceeds the "
+            "%zu-byte scratch; use gk_get_state\n",
+            n, n * sizeof(double), NOALLOC_SCRATCH_BYTES);
+    return GK_ERR_ARG;
+  }
+  double *st = nullptr;
+  int rc = scratch_on(s->device, &st);
+  if (rc != GK_OK) return rc;
+  const size_t fs = s->fs;
+  if (prim || prim_prev) fill_x64_ghosts(s);
+  for (int m = 0; m < 6; m++) {
+    if (prim && prim[m] && (rc = copy_out_scratch(s, st, prim[m], s->X[s->cur] + m * fs)) != GK_OK)
+      return rc;
+    if (prim_prev && prim_prev[m] &&
+        (rc = copy_out_scratch(s, st, prim_prev[m], s->X[1 - s->cur] + m * fs)) != GK_OK)
+      return rc;
+  }
+  for (int m = 0; m < 3; m++) {
+    if (bface && bface[m] && (rc = copy_out_scratch(s, st, bface[m], s->B[s->b_0] + m * fs)) != GK_OK)
+      return rc;
+    if (bface_prev && bface_prev[m] &&
+        (rc = copy_out_scratch(s, st, bface_prev[m], s->B[s->b_m1] + m * fs)) != GK_OK)
       return rc;
   }
   HIPCHK(hipStreamSynchronize(s->stream));

@@ -154,6 +154,20 @@ int gk_step_until(gk_solver *s, int n, double t_stop, double t_event, int *done)
 int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], double *prim_prev[6],
                  double *bface_prev[3]);
 
+/* gk_get_state without any allocation (no counterpart in the reference).
+   For callers that cannot allocate device or pinned memory: code under HIP
+   graph capture, tight serving loops, harnesses that forbid it.  gk_get_state
+   allocates a device staging field and a pinned host buffer on its first
+   call; this variant stages each field through a fixed 32 MiB device array
+   that the library holds on every GPU it runs on until the process ends,
+   and copies into the caller's arrays directly.  Same arguments and results
+   as gk_get_state; GK_ERR_ARG for a grid whose reference-layout field
+   exceeds the array ((ni+9)(nj+9)(nk+9) > 4194304 points, e.g. above
+   152^3 cells).  Calls that overlap on one GPU (several threads or streams)
+   share the array and race: serialise them. */
+int gk_get_state_noalloc(gk_solver *s, double *prim[6], double *bface[3], double *prim_prev[6],
+                         double *bface_prev[3]);
+
 /* scalars: [0] dt of the last step, [1] dt0, [2] time_sim, [3] max|div B|,
    [4] next CFL time step, [5] non-finite flag */
 int gk_get_scalars(gk_solver *s, double out[6]);
