@@ -234,6 +234,14 @@ def run_ensemble(args):
     t0 = time.perf_counter()
     B.step(steps)
     el_b = time.perf_counter() - t0
+    # per-stage launch times of the batch (timed launches, no graphs): where
+    # a batched step spends its time, and the launches' sum against the
+    # step's wall time (the rest is gaps between launches)
+    B.enable_timing(True)
+    B.set_graphs(False)
+    B.step(12)
+    ens_kt = B.kernel_times()
+    B.enable_timing(False)
     B.close()
     log(f"bench: ensemble one grid at a time, {steps} steps")
     t0 = time.perf_counter()
@@ -246,6 +254,9 @@ def run_ensemble(args):
             "batched_cell_updates_per_s": cells * steps / el_b,
             "one_at_a_time_cell_updates_per_s": cells * steps / el_u,
             "speedup": el_u / el_b, "batched_s": el_b, "one_at_a_time_s": el_u,
+            "batched_ms_per_step": el_b * 1e3 / steps,
+            "batched_kernel_ms_per_step": {k: round(v, 4) for k, v in ens_kt.items()},
+            "batched_kernel_sum_ms": round(sum(ens_kt.values()), 4),
             "sizes": "drawn from " + str(ENSEMBLE_SIZES)}
 
 
@@ -280,20 +291,53 @@ def run_serving(args, preparer):
             "throughput_ratio": srv["cell_updates_per_s"] / pl["cell_updates_per_s"]}
 
 
-def run_reference_cpu(args, threads):
-    """The compiled reference (oracle/_ref/ref_driver_cart, the reference's
-    own sources built where they lie) on a bounded sample of the same
+def host_info():
+    """The host CPU the baselines ran on: logical CPUs, CPUs this process may
+    use, and lscpu's model, sockets, cores per socket and threads per core."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+        if "sockets" in info and "cores_per_socket" in info:
+            info["physical_cores"] = int(info["sockets"]) * int(info["cores_per_socket"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+REF_BUILDS = {
+    # oracle/Makefile: the reference's own sources where they lie
+    "release": ("ref_driver_cart_release",
+                "gcc -O3 -march=native -mtune=native -ffast-math -ftree-vectorize "
+                "-fomit-frame-pointer -fopenmp (the reference's Release flags, CMakeLists.txt:45-55)"),
+    "o2": ("ref_driver_cart", "gcc -O2 -ffp-contract=off -fopenmp (the fixture build)"),
+}
+
+
+def run_reference_cpu(args, threads, build):
+    """One build of the compiled reference on a bounded sample of the same
     generator: runs of 1 and 1 + K steps, whose difference cancels set-up and
     output.  None when the binary is absent or does not run."""
     import subprocess
     import tempfile
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver_cart")
+    name, flags = REF_BUILDS[build]
+    exe = os.path.join(ROOT, "oracle", "_ref", name)
     if not os.path.exists(exe):
         return None
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from refrun import YAML
     n, k = args.ref_n, args.ref_steps
-    log(f"bench: compiled reference {n}^3, 1 and {1 + k} steps, {threads} threads")
+    log(f"bench: compiled reference ({build}) {n}^3, 1 and {1 + k} steps, {threads} threads")
     wall = {}
     try:
         for nt in (1, 1 + k):
@@ -307,18 +351,18 @@ def run_reference_cpu(args, threads):
                                    capture_output=True, text=True, timeout=600)
                 wall[nt] = time.perf_counter() - t0
                 if r.returncode != 0:
-                    log(f"bench: compiled reference failed ({r.returncode}); port only")
+                    log(f"bench: compiled reference ({build}) failed ({r.returncode})")
                     return None
     except (OSError, subprocess.SubprocessError) as e:
-        log(f"bench: compiled reference did not run ({e}); port only")
+        log(f"bench: compiled reference ({build}) did not run ({e})")
         return None
     per_step = (wall[1 + k] - wall[1]) / k
-    log(f"bench: compiled reference {per_step:.3f} s/step")
+    log(f"bench: compiled reference ({build}) {per_step:.3f} s/step")
     return {"value": n ** 3 / per_step, "unit": "cell-updates/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"{n}^3 cells of the same generator, the compiled reference (its own "
-                      f"sources, gcc -O2 -fopenmp) run for 1 and {1 + k} AB2 steps; value from "
-                      f"the difference of the two wall times"}
+            "kind": "reference", "build": flags,
+            "sample": f"{n}^3 cells of the same generator (sinusoid + uniform B0 + over-pressure "
+                      f"sphere), the compiled reference run for 1 and {1 + k} AB2 steps; value "
+                      f"from the difference of the two wall times"}
 
 
 def run_cpu(args):
@@ -347,11 +391,22 @@ def run_cpu(args):
             "kind": "port",
             "sample": f"{n}^3 cells of the same generator, {steps} AB2 steps after 1 warm-up, "
                       f"oracle/liboracle_fast.so (-O3 -march=native, OpenMP)"}
-    ref = None if args.no_ref_cpu else run_reference_cpu(args, threads)
-    if ref is None:
+    host = host_info()
+    refs = {} if args.no_ref_cpu else {b: run_reference_cpu(args, threads, b) for b in REF_BUILDS}
+    refs = {b: r for b, r in refs.items() if r is not None}
+    if not refs:
+        port["host"] = host
         return port
-    ref["port"] = port
-    return ref
+    # the reported baseline is the faster build of the reference on these cores
+    best = max(refs, key=lambda b: refs[b]["value"])
+    out = dict(refs[best])
+    out["builds"] = {b: {"value": r["value"], "build": r["build"]} for b, r in refs.items()}
+    out["host"] = host
+    out["port"] = port
+    out["note"] = ("the reference keeps 261 full-grid FP64 fields and runs several loops of "
+                   "the step serially (the magnetic stress has no OpenMP), so it scales poorly "
+                   "with threads; the port fuses the step into a few parallel passes")
+    return out
 
 
 def main():
@@ -380,6 +435,8 @@ def main():
     ap.add_argument("--ring-dims", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--ring-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--only", choices=["serve", "ensemble"], default=None,
+                    help="run one leg alone (profiling)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
@@ -394,6 +451,17 @@ def main():
         return
     if args.ring_only:
         print(json.dumps({"pole_ring": run_ring(args)}), flush=True)
+        return
+    if args.only in ("serve", "ensemble"):  # one leg alone (profiling, tuning)
+        sys.path.insert(0, PKG)
+        if args.only == "serve":
+            import serve
+            preparer = serve.Preparer(args.serve_workers or None)
+            out = {"serving": run_serving(args, preparer)}
+            preparer.close()
+        else:
+            out = {"ensemble": run_ensemble(args)}
+        print(json.dumps(out), flush=True)
         return
     preparer = None
     if not args.no_serve:

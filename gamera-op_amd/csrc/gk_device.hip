@@ -83,6 +83,45 @@ struct B0c {
   double lor[3];
 };
 
+// Background field given per point (b0_mode 2): B0A_COUNT FP64 fields of the
+// solver's field stride, in the device layout.  Faces: for each direction D
+// the face averages b1 b2 b3 and the face normal bn (the reference's
+// B0Aface*_b1..bn, src/solver_func.c:1883-1934); edges: for each direction D
+// the two components the edge field reads, (D+1)%3 then (D+2)%3
+// (B0Aedge*, :1935-1980); the cell-centre field B0_b1..3; the Lorentz force
+// dmagsf_G (:1981-2077).  Null when B0 is zero or the same at every point
+// (b0_mode 0 / 1: the B0c constants).  Kernels that read B0 take a template
+// flag B0V (per-point arrays) so the constant form keeps its registers.
+enum { B0A_FACE = 0, B0A_EDGE = 12, B0A_CELL = 18, B0A_LOR = 21, B0A_COUNT = 24 };
+
+// B0 at a face of direction D (b1 b2 b3 bn)
+template <int D, typename R, bool B0V>
+__device__ __forceinline__ void face_b0(const B0c &b0c, const double *b0a, long fs, long c,
+                                        R b0[4]) {
+  if constexpr (B0V) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) b0[m] = (R)b0a[(B0A_FACE + 4 * D + m) * fs + c];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 4; m++) b0[m] = (R)b0c.face[D][m];
+  }
+}
+
+// B0 at an edge of direction D (the three components; the one along the
+// edge is not read)
+template <int D, bool B0V>
+__device__ __forceinline__ void edge_b0(const B0c &b0c, const double *b0a, long fs, long c,
+                                        double e0[3]) {
+  if constexpr (B0V) {
+    e0[D] = 0.0;
+    e0[(D + 1) % 3] = b0a[(B0A_EDGE + 2 * D) * fs + c];
+    e0[(D + 2) % 3] = b0a[(B0A_EDGE + 2 * D + 1) * fs + c];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; m++) e0[m] = b0c.edge[D][m];
+  }
+}
+
 // Device-resident scalars.  The fused reductions (next CFL dt, max |div B|)
 // land in NSLOT slots chosen by block coordinates, so that tens of thousands
 // of blocks do not all serialise on one atomic word; k_begin folds the slots.
@@ -336,13 +375,10 @@ __device__ __forceinline__ R entropy_over(R ps, R rho) {
 // Fo: mass, momentum 1..3, energy, entropy, magnetic stress 1..3.
 template <int D, typename R>
 __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R bl[3],
-                                          const R br[3], const R Bn, const B0c &b0c,
+                                          const R br[3], const R Bn, const R b0[4],
                                           const double CAd, R Fo[9]) {
   const R CA = (R)CAd;
   const R G = (R)GAMMA;
-  R b0[4];
-#pragma unroll
-  for (int m = 0; m < 4; m++) b0[m] = (R)b0c.face[D][m];
   const R rl = L[0], rr = Rs[0], pl = L[4], pr = Rs[4], psl = L[5], psr = Rs[5];
   const R vl[3] = {L[1], L[2], L[3]}, vr[3] = {Rs[1], Rs[2], Rs[3]};
   R csl = qsqrt(qdiv(G * pl, rl)), csr = qsqrt(qdiv(G * pr, rr));
@@ -488,6 +524,7 @@ struct SweepArgs {
   Grid g;
   Geo geo;
   B0c b0;
+  const double *b0a;     // per-point B0 (b0_mode 2), else null
   // state read by the stencil: of type R (FP64 state, or its FP32 copy in
   // the FP32-bulk form)
   const void *X, *Xp;    // 6 fields
@@ -505,7 +542,7 @@ struct SweepArgs {
 // ghost layers on either side).  Each thread reads its eight stencil cells;
 // the points of the launch box are walked k fastest, then j, then i, 256 per
 // block.
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
   if (halted(a.sc)) return;
@@ -547,21 +584,22 @@ __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, c
   for (int m = 0; m < 3; m++)
     recon((const R *)a.M + m * g.fs, (const R *)a.Mp + m * g.fs, c, sd, cc, T, bl[m], br[m]);
 
-  R Fo[9];
-  face_flux<D, R>(L, Rs, bl, br, ((const R *)a.B)[D * g.fs + c], a.b0, g.CA, Fo);
+  R Fo[9], b0[4];
+  face_b0<D, R, B0V>(a.b0, a.b0a, g.fs, c, b0);
+  face_flux<D, R>(L, Rs, bl, br, ((const R *)a.B)[D * g.fs + c], b0, g.CA, Fo);
 #pragma unroll
   for (int m = 0; m < 9; m++) F[m * g.fs + c] = Fo[m];
 }
 
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
-  k_sweep_body<D, R>(a, bx, by, bz);
+  k_sweep_body<D, R, B0V>(a, bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -571,7 +609,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_sweep_body<D, R>(args[arg], bx, by, bz);
+  k_sweep_body<D, R, B0V>(args[arg], bx, by, bz);
 }
 
 // Marching sweep along D = 0 (i) or 1 (j).  Lanes run along k (coalesced),
@@ -588,7 +626,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
 // transposed through LDS, instead of every lane loading its own 8 cells (64
 // rows per load).  Rows of planes outside [flo0, fhi0] only store the face
 // velocities.  The arithmetic is that of k_sweepk.
-template <typename R>
+template <typename R, bool B0V>
 __device__ __forceinline__ void k_kcol_body(const SweepArgs &a, const int bx, const int by,
                                        const int bz) {
   if (halted(a.sc)) return;
@@ -667,20 +705,21 @@ __device__ __forceinline__ void k_kcol_body(const SweepArgs &a, const int bx, co
   V[c] = R(0.5) * (L[3] + Rs[3]);
   V[fs + c] = R(0.5) * (L[1] + Rs[1]);
   if (!full) return;
-  R Fo[9];
-  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], a.b0, g.CA, Fo);
+  R Fo[9], b0[4];
+  face_b0<2, R, B0V>(a.b0, a.b0a, fs, c, b0);
+  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], b0, g.CA, Fo);
 #pragma unroll
   for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
 }
 
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(64) void k_kcol(SweepArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
-  k_kcol_body<R>(a, bx, by, bz);
+  k_kcol_body<R, B0V>(a, bx, by, bz);
 }
 
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(64) void k_kcol_batch(const SweepArgs *__restrict__ args,
                                                   const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -689,7 +728,7 @@ __global__ __launch_bounds__(64) void k_kcol_batch(const SweepArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_kcol_body<R>(args[arg], bx, by, bz);
+  k_kcol_body<R, B0V>(args[arg], bx, by, bz);
 }
 
 // Points of a marching launch: the (row, k) plane of the launch box is walked
@@ -747,7 +786,7 @@ __device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R 
   Rr = pdm_limit(fr, w.x[m][3], w.x[m][2], w.x[m][1], T[16], T[17]);
 }
 
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
   if (halted(a.sc)) return;
@@ -830,8 +869,9 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
       for (int m = 0; m < 3; m++) march_lr(w, 6 + m, T, bl[m], br[m]);
       V[c] = R(0.5) * (L[va] + Rs[va]);
       V[fs + c] = R(0.5) * (L[vb] + Rs[vb]);
-      R Fo[9];
-      face_flux<D, R>(L, Rs, bl, br, bn, a.b0, g.CA, Fo);
+      R Fo[9], b0[4];
+      face_b0<D, R, B0V>(a.b0, a.b0a, fs, c, b0);
+      face_flux<D, R>(L, Rs, bl, br, bn, b0, g.CA, Fo);
 #pragma unroll
       for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
       march_shift(w);
@@ -858,15 +898,15 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
   }
 }
 
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_march(SweepArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_march_body<D, R>(a, bx, by, bz);
+  k_march_body<D, R, B0V>(a, bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -876,7 +916,7 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_march_body<D, R>(args[arg], bx, by, bz);
+  k_march_body<D, R, B0V>(args[arg], bx, by, bz);
 }
 
 // k sweep (D = 2) with the stencil staged in LDS.  A block is one wave
@@ -888,7 +928,7 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
 constexpr int KSPAN = 72;
 constexpr int CELL_CHUNK = 1;  // cells each thread of the cell update walks along i
 
-template <typename R>
+template <typename R, bool B0V>
 __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, const int by,
                                          const int bz) {
   if (halted(a.sc)) return;
@@ -983,21 +1023,22 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
   for (int m = 0; m < 3; m++) lr(6 + m, bl[m], br[m]);
   V[c] = R(0.5) * (L[3] + Rs[3]);
   V[fs + c] = R(0.5) * (L[1] + Rs[1]);
-  R Fo[9];
-  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], a.b0, g.CA, Fo);
+  R Fo[9], b0[4];
+  face_b0<2, R, B0V>(a.b0, a.b0a, fs, c, b0);
+  face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], b0, g.CA, Fo);
 #pragma unroll
   for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
 }
 
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_sweepk(SweepArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<false>(gx, gy, gz, bx, by, bz)) return;
-  k_sweepk_body<R>(a, bx, by, bz);
+  k_sweepk_body<R, B0V>(a, bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -1007,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *__restric
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_sweepk_body<R>(args[arg], bx, by, bz);
+  k_sweepk_body<R, B0V>(args[arg], bx, by, bz);
 }
 
 // Edge EMF from the edge velocities (va, vb), the two reconstructed face-B
@@ -1054,6 +1095,7 @@ struct EmfArgs {
   Grid g;
   Geo geo;
   B0c b0;
+  const double *b0a;   // per-point B0 (b0_mode 2), else null
   const void *X, *Xp;  // of type R (FP64 state or its FP32 copy)
   const void *B, *Bp;
   const void *V;  // 2 fields of R: face-averaged velocity (from the feeding sweep)
@@ -1148,7 +1190,7 @@ struct EmfWaves {
 // staged in LDS (current and half-step values), and each lane reads its
 // 8-cell stencil from there instead of loading it.  Whole 64-edge tiles only;
 // the unstaged form covers the rest.
-template <int D, typename R, bool STAGED = false>
+template <int D, typename R, bool STAGED, bool B0V>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
   if (halted(a.sc)) return;
@@ -1308,7 +1350,9 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       rsum = u0 + w0 + u1 + w1;
     else
       rsum = u0 + u1 + w0 + w1;
-    E[c] = emf_formula<D, R>(va, vb, b1l, b1r, b2l, b2r, rsum, a.b0.edge[D], g.CA);
+    double e0[3];
+    edge_b0<D, B0V>(a.b0, a.b0a, fs, c, e0);
+    E[c] = emf_formula<D, R>(va, vb, b1l, b1r, b2l, b2r, rsum, e0, g.CA);
     u0 = w0;
     u1 = w1;
     if (VW) {
@@ -1321,23 +1365,23 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   }
 }
 
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_emf_march_body<D, R>(a, bx, by, bz);
+  k_emf_march_body<D, R, false, B0V>(a, bx, by, bz);
 }
 
 // staged form over whole 64-edge tiles along k (E_i, E_j): one wave per block
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(64) void k_emf_stage(EmfArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_emf_march_body<D, R, true>(a, bx, by, bz);
+  k_emf_march_body<D, R, true, B0V>(a, bx, by, bz);
 }
 
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restrict__ args,
                                                        const WorkItem *__restrict__ work,
                                                        int nwork) {
@@ -1347,11 +1391,11 @@ __global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restric
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R, true>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, true, B0V>(args[arg], bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <int D, typename R>
+template <int D, typename R, bool B0V>
 __global__ __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march_batch(const EmfArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
@@ -1362,7 +1406,7 @@ __global__ __launch_bounds__(256)
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, false, B0V>(args[arg], bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1457,6 +1501,7 @@ struct CellArgs {
   Grid g;
   Geo geo;
   B0c b0;
+  const double *b0a;       // per-point B0 (b0_mode 2), else null
   int spherical;
   const double *X, *Xp;    // X^n, X^{n-1}
   double *Xn;              // X^{n+1} (aliases Xp: each cell reads before it writes)
@@ -1478,7 +1523,7 @@ struct CellArgs {
 // src/solver_func.c:2117-2135; spherical sources
 // src/spherical/spherical_general.c:605-688) plus the next step's CFL dt and
 // |div B| (src/solver_func.c:255-343).
-template <typename R>
+template <typename R, bool B0V>
 __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, const int by,
                                        const int bz) {
   if (halted(a.sc)) return;
@@ -1529,8 +1574,27 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
         dU[m] = d == 0 ? v : dU[m] + v;
       }
     }
+    // background field at the cell: Lorentz force, cell-centre B0 and the
+    // face normals of the div B check
+    double lor[3], cb0[3], nb0[3][2];
+    if constexpr (B0V) {
 #pragma unroll
-    for (int m = 0; m < 3; m++) dU[6 + m] += a.b0.lor[m];
+      for (int m = 0; m < 3; m++) {
+        lor[m] = a.b0a[(B0A_LOR + m) * g.fs + c];
+        cb0[m] = a.b0a[(B0A_CELL + m) * g.fs + c];
+        nb0[m][0] = a.b0a[(B0A_FACE + 5 * m) * g.fs + c];
+        nb0[m][1] = a.b0a[(B0A_FACE + 5 * m) * g.fs + c + g.st[m]];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        lor[m] = a.b0.lor[m];
+        cb0[m] = a.b0.cell[m];
+        nb0[m][0] = nb0[m][1] = a.b0.face[m][m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) dU[6 + m] += lor[m];
 
     double x[6];
 #pragma unroll
@@ -1580,7 +1644,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
         double bp = (double)((const R *)a.Mp)[m * g.fs + c];
         bh[m] = bold[m] + cc * (bold[m] - bp);
       }
-      const double *c0 = a.b0.cell;
+      const double *c0 = cb0;
       double Bsq = bh[0] * bh[0] + bh[1] * bh[1] + bh[2] * bh[2] +
                    2 * (bh[0] * c0[0] + bh[1] * c0[1] + bh[2] * c0[2]);
       double M22b = (0.5 * Bsq - (bh[1] + c0[1]) * (bh[1]) - bh[1] * c0[1]);
@@ -1618,7 +1682,7 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     // time-centred total field
     double mt[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) mt[m] = 0.5 * (bnew[m] + bold[m]) + a.b0.cell[m];
+    for (int m = 0; m < 3; m++) mt[m] = 0.5 * (bnew[m] + bold[m]) + cb0[m];
     double alf = (mt[0] * mt[0] + mt[1] * mt[1] + mt[2] * mt[2]) / rhof / (CA * CA);
     double perp = 1 / (1 + alf);
     double dva = alf * (-dt * dU[0]);
@@ -1653,15 +1717,14 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
       if (a.Mn32) a.Mn32[m * g.fs + c] = (float)bnew[m];
     }
     // next step's CFL dt from the new state (magtot = cell B + B0)
-    double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + a.b0.cell[0],
-                          bnew[1] + a.b0.cell[1], bnew[2] + a.b0.cell[2], dxi, dxj, dxk, CA);
+    double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + cb0[0], bnew[1] + cb0[1],
+                          bnew[2] + cb0[2], dxi, dxj, dxk, CA);
     const bool red = !(j >= a.skip[0] && j <= a.skip[1]) && !(j >= a.skip[2] && j <= a.skip[3]);
     if (red && dcell == dcell) dtc = fmin(dtc, dcell);  // fmin in the reference ignores NaN
     // div B of the new face field (with the B0 face normals)
-    const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
-    double db = ((bf[0][1] + f0[0]) * A1[0] - (bf[0][0] + f0[0]) * A0[0]) / V +
-                ((bf[1][1] + f1[1]) * A1[1] - (bf[1][0] + f1[1]) * A0[1]) / V +
-                ((bf[2][1] + f2[2]) * A1[2] - (bf[2][0] + f2[2]) * A0[2]) / V;
+    double db = ((bf[0][1] + nb0[0][1]) * A1[0] - (bf[0][0] + nb0[0][0]) * A0[0]) / V +
+                ((bf[1][1] + nb0[1][1]) * A1[1] - (bf[1][0] + nb0[1][0]) * A0[1]) / V +
+                ((bf[2][1] + nb0[2][1]) * A1[2] - (bf[2][0] + nb0[2][0]) * A0[2]) / V;
     const double adb = fabs(db);
     if (adb != adb)
       bad = true;
@@ -1672,17 +1735,17 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
   if (bad) atomicOr(&a.sc->nanflag, 1u);
 }
 
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_cell(CellArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   (void)gy;
   by = 0;
   if (!xcd_patch(gx, gz, bx, bz)) return;
-  k_cell_body<R>(a, bx, by, bz);
+  k_cell_body<R, B0V>(a, bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <typename R>
+template <typename R, bool B0V>
 __global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -1692,7 +1755,7 @@ __global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_cell_body<R>(args[arg], bx, by, bz);
+  k_cell_body<R, B0V>(args[arg], bx, by, bz);
 }
 
 // CFL dt and |div B| of the current state (prologue; src/solver_func.c:255-343)
@@ -1700,12 +1763,14 @@ struct DtArgs {
   Grid g;
   Geo geo;
   B0c b0;
+  const double *b0a;   // per-point B0 (b0_mode 2), else null
   const double *X, *M, *B;
   const double *dxkj;  // j factor of the phi width in the CFL dt
   int j0, j1;          // rows reduced (blockIdx.y covers them from j0)
   Scal *sc;
 };
 
+template <bool B0V>
 __device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const int by,
                                          const int bz) {
   if (halted(a.sc)) return;
@@ -1717,9 +1782,18 @@ __device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const i
   double dtc = INFINITY, divb = 0.0;
   if (k <= g.e[2] && j <= a.j1) {
     const long c = i * g.st[0] + j * g.st[1] + k;
-    double b[3];
+    double b[3], nb0[3][2];
 #pragma unroll
-    for (int m = 0; m < 3; m++) b[m] = a.M[m * g.fs + c] + a.b0.cell[m];
+    for (int m = 0; m < 3; m++) {
+      if constexpr (B0V) {
+        b[m] = a.M[m * g.fs + c] + a.b0a[(B0A_CELL + m) * g.fs + c];
+        nb0[m][0] = a.b0a[(B0A_FACE + 5 * m) * g.fs + c];
+        nb0[m][1] = a.b0a[(B0A_FACE + 5 * m) * g.fs + c + g.st[m]];
+      } else {
+        b[m] = a.M[m * g.fs + c] + a.b0.cell[m];
+        nb0[m][0] = nb0[m][1] = a.b0.face[m][m];
+      }
+    }
     // phi width: the ring-widened one on averaged rows (get_dt_ring,
     // src/spherical/spherical_general.c:700-735); the j factor carries it
     const double dxk = G.g[GK_G_DXK][0][i] * a.dxkj[j] * G.g[GK_G_DXK][2][k];
@@ -1728,30 +1802,31 @@ __device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const i
                  g.CA);
     if (!(dtc == dtc)) dtc = INFINITY;
     const double V = geo3(G, GK_G_VOL, i, j, k);
-    const double *f0 = a.b0.face[0], *f1 = a.b0.face[1], *f2 = a.b0.face[2];
     const double *Bf = a.B;
-    double db = ((Bf[c + g.st[0]] + f0[0]) * geo3(G, GK_G_FACE_I, i + 1, j, k) -
-                 (Bf[c] + f0[0]) * geo3(G, GK_G_FACE_I, i, j, k)) / V +
-                ((Bf[g.fs + c + g.st[1]] + f1[1]) * geo3(G, GK_G_FACE_J, i, j + 1, k) -
-                 (Bf[g.fs + c] + f1[1]) * geo3(G, GK_G_FACE_J, i, j, k)) / V +
-                ((Bf[2 * g.fs + c + 1] + f2[2]) * geo3(G, GK_G_FACE_K, i, j, k + 1) -
-                 (Bf[2 * g.fs + c] + f2[2]) * geo3(G, GK_G_FACE_K, i, j, k)) / V;
+    double db = ((Bf[c + g.st[0]] + nb0[0][1]) * geo3(G, GK_G_FACE_I, i + 1, j, k) -
+                 (Bf[c] + nb0[0][0]) * geo3(G, GK_G_FACE_I, i, j, k)) / V +
+                ((Bf[g.fs + c + g.st[1]] + nb0[1][1]) * geo3(G, GK_G_FACE_J, i, j + 1, k) -
+                 (Bf[g.fs + c] + nb0[1][0]) * geo3(G, GK_G_FACE_J, i, j, k)) / V +
+                ((Bf[2 * g.fs + c + 1] + nb0[2][1]) * geo3(G, GK_G_FACE_K, i, j, k + 1) -
+                 (Bf[2 * g.fs + c] + nb0[2][0]) * geo3(G, GK_G_FACE_K, i, j, k)) / V;
     divb = fabs(db);
   }
   block_reduce(dtc, divb, a.sc, bx * 31 + by * 7 + bz);
 }
 
+template <bool B0V>
 __global__ __launch_bounds__(256) void k_dt(DtArgs a) {
-  k_dt_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+  k_dt_body<B0V>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // one launch for many grids: block w runs work-list entry w
+template <bool B0V>
 __global__ __launch_bounds__(256) void k_dt_batch(const DtArgs *__restrict__ args,
     const WorkItem *__restrict__ work, int nwork) {
   int w;
   if (!xcd_item(nwork, w)) return;
   const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-  k_dt_body(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+  k_dt_body<B0V>(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
       __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
 }
 
@@ -2539,6 +2614,7 @@ struct gk_solver {
   Geo geo;
   B0c b0;
   int device;          // HIP device the solver was created on
+  double *b0a;         // per-point background field (b0_mode 2), else null
   int spherical;
   int bc[2][6];
   int ni, nj, nk;
@@ -2597,8 +2673,98 @@ struct gk_solver {
     }                                                                          \
   } while (0)
 
+// Device memory of solvers and batches is stream ordered: hipMallocAsync /
+// hipFreeAsync on the owner's stream, from the device's default pool, which
+// keeps what is freed for reuse.  Destroying a solver or rebuilding a
+// batch's tables then never waits for work that other solvers and batches
+// have in flight (a plain hipFree synchronises the whole device, so the
+// serving loop's retirements used to stall on the quantum running beside
+// them).  Pinned host buffers are recycled for the same reason (hipHostFree
+// synchronises the device too).
+constexpr int MAX_DEV = 64;
+static std::mutex pool_mu;
+static bool pool_kept[MAX_DEV];
+
+static void keep_pool(int dev) {
+  if (dev < 0 || dev >= MAX_DEV) return;
+  std::lock_guard<std::mutex> lock(pool_mu);
+  if (pool_kept[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  pool_kept[dev] = true;
+}
+
+// streams are recycled too: destroying one may wait for the device
+static std::mutex stream_mu;
+static std::vector<std::pair<int, hipStream_t>> stream_free;  // (device, stream)
+
+static hipError_t stream_get(int dev, hipStream_t *out) {
+  {
+    std::lock_guard<std::mutex> lock(stream_mu);
+    for (size_t q = 0; q < stream_free.size(); q++)
+      if (stream_free[q].first == dev) {
+        *out = stream_free[q].second;
+        stream_free.erase(stream_free.begin() + q);
+        return hipSuccess;
+      }
+  }
+  return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+}
+
+static void stream_put(int dev, hipStream_t st) {
+  if (!st) return;
+  std::lock_guard<std::mutex> lock(stream_mu);
+  stream_free.push_back({dev, st});
+}
+
+static std::mutex pin_mu;
+static std::vector<std::pair<void *, size_t>> pin_free;  // recycled buffers
+static std::vector<std::pair<void *, size_t>> pin_live;  // handed out
+static size_t pin_cached = 0;
+constexpr size_t PIN_CACHE_MAX = size_t(2) << 30;
+
+static void *pinned_get(size_t n) {
+  std::lock_guard<std::mutex> lock(pin_mu);
+  int best = -1;
+  for (int q = 0; q < (int)pin_free.size(); q++)
+    if (pin_free[q].second >= n && (best < 0 || pin_free[q].second < pin_free[best].second))
+      best = q;
+  void *p = nullptr;
+  size_t sz = n;
+  if (best >= 0) {
+    p = pin_free[best].first;
+    sz = pin_free[best].second;
+    pin_cached -= sz;
+    pin_free.erase(pin_free.begin() + best);
+  } else if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+    return nullptr;
+  }
+  pin_live.push_back({p, sz});
+  return p;
+}
+
+static void pinned_put(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lock(pin_mu);
+  for (size_t q = 0; q < pin_live.size(); q++)
+    if (pin_live[q].first == p) {
+      const size_t sz = pin_live[q].second;
+      pin_live.erase(pin_live.begin() + q);
+      if (pin_cached + sz <= PIN_CACHE_MAX) {
+        pin_free.push_back({p, sz});
+        pin_cached += sz;
+      } else {
+        (void)hipHostFree(p);
+      }
+      return;
+    }
+}
+
 static int dev_alloc_bytes(gk_solver *s, void **p, size_t nbytes) {
-  if (hipMalloc(p, nbytes) != hipSuccess) return GK_ERR_NOMEM;
+  if (hipMallocAsync(p, nbytes, s->stream) != hipSuccess) return GK_ERR_NOMEM;
   s->bytes += nbytes;
   if (hipMemsetAsync(*p, 0, nbytes, s->stream) != hipSuccess) return GK_ERR_DEVICE;
   return GK_OK;
@@ -2622,13 +2788,32 @@ static int field_alloc(gk_solver *s, void **p, size_t nbytes, size_t esz) {
   return GK_OK;
 }
 
+static int copy_in(gk_solver *s, double *dst, const double *src);
+static int release_bounce(gk_solver *s);
+
+// small host -> device upload through a recycled pinned buffer (a copy from
+// pageable memory can wait behind other streams' work); waits for it
+static hipError_t upload(gk_solver *s, void *dst, const void *src, size_t bytes) {
+  void *pin = pinned_get(bytes);
+  if (!pin) return hipErrorOutOfMemory;
+  memcpy(pin, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  pinned_put(pin);
+  return e;
+}
+
 extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if (!d || !out) return GK_ERR_ARG;
   *out = nullptr;
   if (d->ni < 5 || d->nj < 5 || d->nk < 5) return GK_ERR_ARG;
   if (d->coord != 0 && d->coord != 2) return GK_ERR_ARG;
   if (d->precision != 0 && d->precision != 1) return GK_ERR_ARG;
-  if (d->b0_mode != 0 && d->b0_mode != 1) return GK_ERR_ARG;
+  if (d->b0_mode < 0 || d->b0_mode > 2) return GK_ERR_ARG;
+  static_assert(GK_B0_FIELDS == B0A_COUNT, "per-point B0 fields");
+  if (d->b0_mode == 2)
+    for (int q = 0; q < GK_B0_FIELDS; q++)
+      if (!d->b0_field[q]) return GK_ERR_ARG;
   // floors: >= 0 and finite (0 selects the reference's initial value);
   // gravity finite
   if (!(d->rho_floor >= 0.0 && d->rho_floor < INFINITY) ||
@@ -2679,9 +2864,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     delete s;
     return GK_ERR_DEVICE;
   }
+  keep_pool(s->device);
   // every upload of the set-up runs on the solver's own stream, so creating
   // a solver does not wait for work other solvers or batches have in flight
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (stream_get(s->device, &s->stream) != hipSuccess) {
     delete s;
     return GK_ERR_DEVICE;
   }
@@ -2757,12 +2943,11 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   };
   if (!cls_ok) return fail(GK_ERR_ARG);
   if ((rc = dev_alloc(s, &s->tables, h.size())) != GK_OK) return fail(rc);
-  if (hipMemcpyAsync(s->tables, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                     s->stream) != hipSuccess)
+  if (upload(s, s->tables, h.data(), h.size() * sizeof(double)) != hipSuccess)
     return fail(GK_ERR_DEVICE);
-  if (hipMalloc((void **)&s->itables, hi.size() * sizeof(int)) != hipSuccess) return fail(GK_ERR_NOMEM);
-  if (hipMemcpyAsync(s->itables, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice,
-                     s->stream) != hipSuccess)
+  if (hipMallocAsync((void **)&s->itables, hi.size() * sizeof(int), s->stream) != hipSuccess)
+    return fail(GK_ERR_NOMEM);
+  if (upload(s, s->itables, hi.data(), hi.size() * sizeof(int)) != hipSuccess)
     return fail(GK_ERR_DEVICE);
   for (int q = 0; q < GK_G_COUNT; q++)
     for (int a = 0; a < 3; a++) s->geo.g[q][a] = s->tables + off_g[q][a];
@@ -2805,8 +2990,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
       }
     }
     if ((rc = dev_alloc(s, &s->ring_tab, hr.size())) != GK_OK) return fail(rc);
-    if (hipMemcpyAsync(s->ring_tab, hr.data(), hr.size() * sizeof(double), hipMemcpyHostToDevice,
-                       s->stream) != hipSuccess)
+    if (upload(s, s->ring_tab, hr.data(), hr.size() * sizeof(double)) != hipSuccess)
       return fail(GK_ERR_DEVICE);
     s->dxkj = s->ring_tab + 2 * nk;
   }
@@ -2816,8 +3000,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     for (size_t q = 0; q < h.size(); q++) hf[q] = (float)h[q];
     if ((rc = dev_alloc_bytes(s, (void **)&s->tables_f, hf.size() * sizeof(float))) != GK_OK)
       return fail(rc);
-    if (hipMemcpyAsync(s->tables_f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice,
-                       s->stream) != hipSuccess)
+    if (upload(s, s->tables_f, hf.data(), hf.size() * sizeof(float)) != hipSuccess)
       return fail(GK_ERR_DEVICE);
     for (int a = 0; a < 3; a++) s->geo.recwf[a] = s->tables_f + off_w[a];
   }
@@ -2843,17 +3026,28 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
     if ((rc = field_alloc(s, &s->V[d2], 2 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
   }
   if ((rc = field_alloc(s, &s->E, 3 * fs * s->esz, s->esz)) != GK_OK) return fail(rc);
-  if (hipMalloc((void **)&s->sc, sizeof(Scal)) != hipSuccess) return fail(GK_ERR_NOMEM);
-  if (hipHostMalloc((void **)&s->ctl_host, 2 * sizeof(long long), hipHostMallocDefault) !=
-      hipSuccess) {
-    s->ctl_host = nullptr;
-    return fail(GK_ERR_NOMEM);
+  if (d->b0_mode == 2) {  // the background field per point (FP64 in both forms)
+    void *b0a = nullptr;
+    if ((rc = field_alloc(s, &b0a, (size_t)B0A_COUNT * fs * sizeof(double), sizeof(double))) !=
+        GK_OK)
+      return fail(rc);
+    s->b0a = (double *)b0a;
+    for (int q = 0; q < B0A_COUNT; q++)
+      if ((rc = copy_in(s, s->b0a + q * fs, d->b0_field[q])) != GK_OK) return fail(rc);
   }
+  if (hipMallocAsync((void **)&s->sc, sizeof(Scal), s->stream) != hipSuccess)
+    return fail(GK_ERR_NOMEM);
+  // pinned: steps taken and non-finite flag of the last run, then room for a
+  // copy of the device scalars (gk_get_scalars)
+  if (!(s->ctl_host = (long long *)pinned_get(2 * sizeof(long long) + sizeof(Scal))))
+    return fail(GK_ERR_NOMEM);
   Scal z{};
   z.budget = -1;
   z.t_stop = INFINITY;
   z.t_event = INFINITY;
-  if (hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+  memcpy(s->ctl_host + 2, &z, sizeof z);  // through pinned memory
+  if (hipMemcpyAsync(s->sc, s->ctl_host + 2, sizeof(Scal), hipMemcpyHostToDevice, s->stream) !=
+      hipSuccess)
     return fail(GK_ERR_DEVICE);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
   s->chunk = 32;
@@ -2862,6 +3056,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   s->b_m1 = 0;
   s->b_0 = 1;
   s->b_n = 2;
+  if ((rc = release_bounce(s)) != GK_OK) return fail(rc);
   *out = s;
   return GK_OK;
 }
@@ -2869,18 +3064,17 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
 extern "C" int gk_destroy(gk_solver *s) {
   if (!s) return GK_OK;
   if (s->batch) return GK_ERR_STATE;  // remove it from its batch first
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  (void)hipFree(s->tables);
-  (void)hipFree(s->tables_f);
-  (void)hipFree(s->ring_tab);
-  (void)hipFree(s->stage);
-  if (s->pin) (void)hipHostFree(s->pin);
-  if (s->ctl_host) (void)hipHostFree(s->ctl_host);
-  (void)hipFree(s->itables);
-  for (void *p : s->field_mem) (void)hipFree(p);
-  (void)hipFree(s->sc);
+  if (s->stream) {
+    (void)hipStreamSynchronize(s->stream);  // the solver's own work only
+    void *dev[] = {s->tables, s->tables_f, s->ring_tab, s->stage, s->itables, s->sc};
+    for (void *p : dev)
+      if (p) (void)hipFreeAsync(p, s->stream);
+    for (void *p : s->field_mem) (void)hipFreeAsync(p, s->stream);
+  }
+  pinned_put(s->pin);
+  pinned_put(s->ctl_host);
   for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  stream_put(s->device, s->stream);
   delete s;
   return GK_OK;
 }
@@ -2893,22 +3087,42 @@ size_t gk_device_bytes(const gk_solver *s) { return s ? s->bytes : 0; }
 static int staging(gk_solver *s, double **out) {
   if (!s->stage) {
     const size_t n = (size_t)s->g.N[0] * s->g.N[1] * s->g.N[2];
-    if (hipMalloc((void **)&s->stage, n * sizeof(double)) != hipSuccess) return GK_ERR_NOMEM;
+    if (hipMallocAsync((void **)&s->stage, n * sizeof(double), s->stream) != hipSuccess)
+      return GK_ERR_NOMEM;
     s->bytes += n * sizeof(double);
   }
   *out = s->stage;
   return GK_OK;
 }
 
+static void host_copy(double *dst, const double *src, size_t n);
+
+// Host -> device: the caller's array is copied into the pinned bounce buffer
+// (a copy from pageable memory can wait behind other streams' work), then to
+// the staging field and re-pitched.
 static int copy_in(gk_solver *s, double *dst, const double *src) {
   double *st;
   int rc = staging(s, &st);
   if (rc != GK_OK) return rc;
   const long rows = (long)s->g.N[0] * s->g.N[1];
   const int NK = s->g.N[2];
-  HIPCHK(hipMemcpyAsync(st, src, rows * NK * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  const size_t n = (size_t)rows * NK;
+  if (!s->pin && !(s->pin = (double *)pinned_get(n * sizeof(double)))) return GK_ERR_NOMEM;
+  HIPCHK(hipStreamSynchronize(s->stream));  // the bounce buffer's last copy is done
+  host_copy(s->pin, src, n);
+  HIPCHK(hipMemcpyAsync(st, s->pin, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
   k_pitch<<<2048, 256, 0, s->stream>>>(st, dst, rows, NK, NK, (int)s->g.st[1]);
   HIPCHK(hipGetLastError());
+  return GK_OK;
+}
+
+// Hand the bounce buffer back to the pool once the transfers that use it are
+// done (a solver keeps none between calls).
+static int release_bounce(gk_solver *s) {
+  if (!s->pin) return GK_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  pinned_put(s->pin);
+  s->pin = nullptr;
   return GK_OK;
 }
 
@@ -2939,10 +3153,7 @@ static int copy_out(gk_solver *s, double *dst, const double *src) {
   const long rows = (long)s->g.N[0] * s->g.N[1];
   const int NK = s->g.N[2];
   const size_t n = (size_t)rows * NK;
-  if (!s->pin && hipHostMalloc((void **)&s->pin, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    s->pin = nullptr;
-    return GK_ERR_NOMEM;
-  }
+  if (!s->pin && !(s->pin = (double *)pinned_get(n * sizeof(double)))) return GK_ERR_NOMEM;
   k_pitch<<<2048, 256, 0, s->stream>>>(src, st, rows, NK, (int)s->g.st[1], NK);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s->pin, st, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -3069,7 +3280,7 @@ extern "C" int gk_set_state(gk_solver *s, const double *const prim[6], const dou
   }
   s->ready = hist ? 2 : 1;
   HIPCHK(hipStreamSynchronize(s->stream));
-  return GK_OK;
+  return release_bounce(s);
 }
 
 extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
@@ -3095,9 +3306,13 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   HIPCHK(hipMemcpyAsync(s->B[s->b_n], s->B[s->b_0], 3 * fs * sizeof(double),
                         hipMemcpyDeviceToDevice, s->stream));
   refresh_copies32(s);
-  Scal cur;
-  HIPCHK(hipMemcpyAsync(&cur, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
+  // scalars through the solver's pinned buffer (pageable copies can wait
+  // behind other streams' work)
+  Scal *pin = (Scal *)(s->ctl_host + 2);
+  HIPCHK(hipMemcpyAsync(pin, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  Scal cur;
+  memcpy(&cur, pin, sizeof cur);
   Scal z{};
   for (int q = 0; q < NSLOT; q++) {
     z.smin[q] = DBL_INF_BITS;
@@ -3107,11 +3322,13 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   z.budget = -1;
   z.t_stop = INFINITY;
   z.t_event = INFINITY;
-  HIPCHK(hipMemcpyAsync(s->sc, &z, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
+  memcpy(pin, &z, sizeof z);
+  HIPCHK(hipMemcpyAsync(s->sc, pin, sizeof(Scal), hipMemcpyHostToDevice, s->stream));
   DtArgs a;
   a.g = s->g;
   a.geo = s->geo;
   a.b0 = s->b0;
+  a.b0a = s->b0a;
   a.X = s->X[s->cur];
   a.M = s->M[s->b_0];
   a.B = s->B[s->b_0];
@@ -3119,12 +3336,16 @@ extern "C" int gk_prologue(gk_solver *s, double dt0_restart) {
   a.j0 = s->g.s[1];
   a.j1 = s->g.e[1];
   a.sc = s->sc;
-  hipLaunchKernelGGL(k_dt, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
+  if (s->b0a)
+    hipLaunchKernelGGL(k_dt<true>, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
+  else
+    hipLaunchKernelGGL(k_dt<false>, box_grid(s->nk, s->nj, s->ni), dim3(64, 4), 0, s->stream, a);
   if (s->ready == 1) {
     hipLaunchKernelGGL(k_set_dt0, dim3(1), dim3(NSLOT), 0, s->stream, s->sc);
   } else {
-    HIPCHK(hipMemcpyAsync(&s->sc->dt0, &dt0_restart, sizeof(double), hipMemcpyHostToDevice,
-                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));  // the pinned scalars above have gone out
+    memcpy(pin, &dt0_restart, sizeof(double));
+    HIPCHK(hipMemcpyAsync(&s->sc->dt0, pin, sizeof(double), hipMemcpyHostToDevice, s->stream));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -3296,6 +3517,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
+    a.b0a = s->b0a;
+  a.b0a = s->b0a;
     a.X = Xs;
     a.Xp = Xps;
     a.M = Ms;
@@ -3380,6 +3603,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
+    a.b0a = s->b0a;
+  a.b0a = s->b0a;
     a.X = Xs;
     a.Xp = Xps;
     a.B = Bs;
@@ -3470,6 +3695,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
+    a.b0a = s->b0a;
+  a.b0a = s->b0a;
     a.spherical = s->spherical;
     a.X = X;
     a.Xp = Xp;
@@ -3563,6 +3790,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     a.g = g;
     a.geo = s->geo;
     a.b0 = s->b0;
+    a.b0a = s->b0a;
+  a.b0a = s->b0a;
     a.X = Xn;
     a.M = Mn;
     a.B = Bn;
@@ -3580,7 +3809,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
   }
 }
 
-template <typename R>
+template <typename R, bool B0V>
 static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 blk(64, 4);
   const int gx = L.grid.x, gy = L.grid.y, gz = L.grid.z;
@@ -3590,18 +3819,18 @@ static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 g1((total + NXCD - 1) / NXCD * NXCD);
   switch (L.kid) {
     case SK_BEGIN: k_begin<<<1, NSLOT, 0, st>>>(L.u.sc); break;
-    case SK_SWEEP_I: k_march<0, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_SWEEP_J: k_march<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_VSTRIP_J: k_sweep<1, R><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_SWEEP_K: k_sweepk<R><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_KREM: k_kcol<R><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_EMF_I: k_emf_march<0, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_J: k_emf_march<1, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_K: k_emf_march<2, R><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_IS: k_emf_stage<0, R><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_JS: k_emf_stage<1, R><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_SWEEP_I: k_march<0, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_J: k_march<1, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_VSTRIP_J: k_sweep<1, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_K: k_sweepk<R, B0V><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_KREM: k_kcol<R, B0V><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_EMF_I: k_emf_march<0, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_J: k_emf_march<1, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_K: k_emf_march<2, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_IS: k_emf_stage<0, R, B0V><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_JS: k_emf_stage<1, R, B0V><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_CT: k_ct<R><<<g1, blk, 0, st>>>(L.u.ct, gx, gy, gz); break;
-    case SK_CELL: k_cell<R><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
+    case SK_CELL: k_cell<R, B0V><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
     case SK_MAGS: k_mags_shell<<<g1, blk, 0, st>>>(L.u.mg, gx, gy, gz); break;
     case SK_POLE_E: k_pole_e<R><<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
@@ -3610,28 +3839,28 @@ static void launch_one(const Launch &L, hipStream_t st) {
     case SK_RING_CURL: k_ring_curl<R><<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
     case SK_RING_FIX: k_ring_fix<<<L.grid, dim3(64), 0, st>>>(L.u.rg); break;
     case SK_RING_MAGS: k_ring_mags<<<L.grid, dim3(256), 0, st>>>(L.u.rg); break;
-    case SK_RING_DT: k_dt<<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt); break;
+    case SK_RING_DT: k_dt<B0V><<<L.grid, dim3(64, 4), 0, st>>>(L.u.dt); break;
   }
 }
 
-template <typename R>
+template <typename R, bool B0V>
 static void launch_batch(int kid, const void *args, const WorkItem *work, int nwork, int nblocks,
                          hipStream_t st) {
   const dim3 blk(64, 4);
   switch (kid) {
     case SK_BEGIN: k_begin_batch<<<nwork, NSLOT, 0, st>>>((Scal *const *)args, nwork); break;
-    case SK_SWEEP_I: k_march_batch<0, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_SWEEP_J: k_march_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_VSTRIP_J: k_sweep_batch<1, R><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_SWEEP_K: k_sweepk_batch<R><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_KREM: k_kcol_batch<R><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_EMF_I: k_emf_march_batch<0, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_J: k_emf_march_batch<1, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_K: k_emf_march_batch<2, R><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_IS: k_emf_stage_batch<0, R><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_JS: k_emf_stage_batch<1, R><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_SWEEP_I: k_march_batch<0, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_J: k_march_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_VSTRIP_J: k_sweep_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_K: k_sweepk_batch<R, B0V><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_KREM: k_kcol_batch<R, B0V><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_EMF_I: k_emf_march_batch<0, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_J: k_emf_march_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_K: k_emf_march_batch<2, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_IS: k_emf_stage_batch<0, R, B0V><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_JS: k_emf_stage_batch<1, R, B0V><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_CT: k_ct_batch<R><<<nblocks, blk, 0, st>>>((const CtArgs *)args, work, nwork); break;
-    case SK_CELL: k_cell_batch<R><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
+    case SK_CELL: k_cell_batch<R, B0V><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
     case SK_GHOST: k_ghost_batch<<<nblocks, blk, 0, st>>>((const GhostArgs *)args, work, nwork); break;
     case SK_MAGS: k_mags_batch<<<nblocks, blk, 0, st>>>((const MagsArgs *)args, work, nwork); break;
     case SK_POLE_E: k_pole_e_batch<R><<<nblocks, dim3(64), 0, st>>>((const RingArgs *)args, work, nwork); break;
@@ -3640,7 +3869,7 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
     case SK_RING_CURL: k_ring_curl_batch<R><<<nblocks, dim3(256), 0, st>>>((const RingArgs *)args, work, nwork); break;
     case SK_RING_FIX: k_ring_fix_batch<<<nblocks, dim3(64), 0, st>>>((const RingArgs *)args, work, nwork); break;
     case SK_RING_MAGS: k_ring_mags_batch<<<nblocks, dim3(256), 0, st>>>((const RingArgs *)args, work, nwork); break;
-    case SK_RING_DT: k_dt_batch<<<nblocks, dim3(64, 4), 0, st>>>((const DtArgs *)args, work, nwork); break;
+    case SK_RING_DT: k_dt_batch<B0V><<<nblocks, dim3(64, 4), 0, st>>>((const DtArgs *)args, work, nwork); break;
   }
 }
 
@@ -3649,10 +3878,11 @@ static void one_step(gk_solver *s) {
   plan_step(s, rot_of(s), plan);
   for (const Launch &L : plan) {
     Timer t(s, sk_name[L.kid]);
+    const bool v = s->b0a != nullptr;
     if (s->precision == 1)
-      launch_one<float>(L, s->stream);
+      v ? launch_one<float, true>(L, s->stream) : launch_one<float, false>(L, s->stream);
     else
-      launch_one<double>(L, s->stream);
+      v ? launch_one<double, true>(L, s->stream) : launch_one<double, false>(L, s->stream);
   }
   const Rot r = rot_next(rot_of(s));
   s->cur = r.cur;
@@ -3738,7 +3968,7 @@ extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], dou
       return rc;
   }
   HIPCHK(hipStreamSynchronize(s->stream));
-  return GK_OK;
+  return release_bounce(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -3757,12 +3987,11 @@ extern "C" int gk_get_state(gk_solver *s, double *prim[6], double *bface[3], dou
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;  // one field of up to 4Mi points
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
-constexpr int MAX_DEVICES = 64;
 static std::mutex scratch_mu;
-static void *scratch_dev[MAX_DEVICES];
+static void *scratch_dev[MAX_DEV];
 
 static int scratch_on(int dev, double **out) {
-  if (dev < 0 || dev >= MAX_DEVICES) return GK_ERR_ARG;
+  if (dev < 0 || dev >= MAX_DEV) return GK_ERR_ARG;
   std::lock_guard<std::mutex> lock(scratch_mu);
   if (!scratch_dev[dev]) {
     int cur = -1;
@@ -3864,9 +4093,12 @@ extern "C" int gk_debug_fetch(gk_solver *s, int which, int field, double *host) 
 
 extern "C" int gk_get_scalars(gk_solver *s, double out[6]) {
   if (!s || !out) return GK_ERR_ARG;
-  Scal h;
-  HIPCHK(hipMemcpyAsync(&h, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
+  // through pinned memory: a pageable copy can wait behind other streams' work
+  Scal *pin = (Scal *)(s->ctl_host + 2);
+  HIPCHK(hipMemcpyAsync(pin, s->sc, sizeof(Scal), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  Scal h;
+  memcpy(&h, pin, sizeof h);
   out[0] = h.dt;
   out[1] = h.dt0;
   out[2] = h.time;
@@ -3883,7 +4115,10 @@ extern "C" int gk_get_scalars(gk_solver *s, double out[6]) {
 
 extern "C" int gk_set_time(gk_solver *s, double t) {
   if (!s) return GK_ERR_ARG;
-  HIPCHK(hipMemcpyAsync(&s->sc->time, &t, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  double *pin = (double *)(s->ctl_host + 2);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *pin = t;
+  HIPCHK(hipMemcpyAsync(&s->sc->time, pin, sizeof(double), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return GK_OK;
 }
@@ -3966,6 +4201,12 @@ extern "C" int gk_get_tuning(const gk_solver *s, int param) {
 // work list covering every grid in the batch (grids of one precision share a
 // launch); the launches of a step are captured in HIP graphs.  Each grid keeps
 // its own dt, time and history, and runs exactly the arithmetic of gk_step.
+constexpr int NGROUP = 4;  // launch groups of a batch (precision x per-point B0)
+
+static int launch_group(const gk_solver *s) {
+  return (s->precision == 1 ? 1 : 0) + (s->b0a ? 2 : 0);
+}
+
 struct gk_batch {
   std::vector<gk_solver *> mem;
   hipStream_t stream = nullptr;
@@ -3973,10 +4214,11 @@ struct gk_batch {
   int phase = 0;  // batch steps since the tables were built, mod 6
   char *dargs = nullptr;
   WorkItem *dwork = nullptr;
-  size_t args_off[2][6][SK_COUNT];
-  int nargs[2][SK_COUNT];
-  size_t work_off[2][SK_COUNT];
-  int nwork[2][SK_COUNT];
+  // launch groups: FP64, FP32-bulk, and each again with per-point B0
+  size_t args_off[NGROUP][6][SK_COUNT];
+  int nargs[NGROUP][SK_COUNT];
+  size_t work_off[NGROUP][SK_COUNT];
+  int nwork[NGROUP][SK_COUNT];
   int use_graphs = 1;
   hipGraphExec_t g1[6] = {};
   hipGraphExec_t g6 = nullptr;
@@ -4002,11 +4244,10 @@ static void batch_free_tables(gk_batch *b) {
     (void)hipGraphExecDestroy(b->g6);
     b->g6 = nullptr;
   }
-  (void)hipFree(b->dargs);
-  (void)hipFree(b->dwork);
-  (void)hipFree(b->dscs);
-  (void)hipFree(b->dflags);
-  if (b->hflags) (void)hipHostFree(b->hflags);
+  void *dev[] = {b->dargs, b->dwork, b->dscs, b->dflags};
+  for (void *p : dev)
+    if (p) (void)hipFreeAsync(p, b->stream);
+  pinned_put(b->hflags);
   b->dargs = nullptr;
   b->dwork = nullptr;
   b->dscs = nullptr;
@@ -4018,11 +4259,11 @@ static void batch_free_tables(gk_batch *b) {
 static int batch_build(gk_batch *b) {
   HIPCHK(hipStreamSynchronize(b->stream));  // nothing in flight uses the old tables
   batch_free_tables(b);
-  std::vector<char> hargs[2][6][SK_COUNT];
-  std::vector<WorkItem> hwork[2][SK_COUNT];
+  std::vector<char> hargs[NGROUP][6][SK_COUNT];
+  std::vector<WorkItem> hwork[NGROUP][SK_COUNT];
   std::vector<Launch> plan;
   for (gk_solver *s : b->mem) {
-    const int gp = s->precision == 1 ? 1 : 0;
+    const int gp = launch_group(s);
     Rot r = rot_of(s);
     for (int p = 0; p < 6; p++) {
       plan_step(s, r, plan);
@@ -4042,7 +4283,7 @@ static int batch_build(gk_batch *b) {
     }
   }
   size_t abytes = 0, nw = 0;
-  for (int gp = 0; gp < 2; gp++)
+  for (int gp = 0; gp < NGROUP; gp++)
     for (int k = 0; k < SK_COUNT; k++) {
       for (int p = 0; p < 6; p++) {
         b->args_off[gp][p][k] = abytes;
@@ -4053,34 +4294,35 @@ static int batch_build(gk_batch *b) {
       b->nwork[gp][k] = k == SK_BEGIN ? b->nargs[gp][k] : (int)hwork[gp][k].size();
       nw += hwork[gp][k].size();
     }
-  if (abytes && hipMalloc((void **)&b->dargs, abytes) != hipSuccess) return GK_ERR_NOMEM;
-  if (nw && hipMalloc((void **)&b->dwork, nw * sizeof(WorkItem)) != hipSuccess) return GK_ERR_NOMEM;
-  for (int gp = 0; gp < 2; gp++)
+  if (abytes && hipMallocAsync((void **)&b->dargs, abytes, b->stream) != hipSuccess)
+    return GK_ERR_NOMEM;
+  if (nw && hipMallocAsync((void **)&b->dwork, nw * sizeof(WorkItem), b->stream) != hipSuccess)
+    return GK_ERR_NOMEM;
+  for (int gp = 0; gp < NGROUP; gp++)
     for (int k = 0; k < SK_COUNT; k++) {
       for (int p = 0; p < 6; p++)
         if (!hargs[gp][p][k].empty())
-          HIPCHK(hipMemcpy(b->dargs + b->args_off[gp][p][k], hargs[gp][p][k].data(),
-                           hargs[gp][p][k].size(), hipMemcpyHostToDevice));
+          HIPCHK(hipMemcpyAsync(b->dargs + b->args_off[gp][p][k], hargs[gp][p][k].data(),
+                                hargs[gp][p][k].size(), hipMemcpyHostToDevice, b->stream));
       if (!hwork[gp][k].empty())
-        HIPCHK(hipMemcpy(b->dwork + b->work_off[gp][k], hwork[gp][k].data(),
-                         hwork[gp][k].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(b->dwork + b->work_off[gp][k], hwork[gp][k].data(),
+                              hwork[gp][k].size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                              b->stream));
     }
   const int nm = (int)b->mem.size();
   if (nm) {
     std::vector<Scal *> hs(nm);
     for (int q = 0; q < nm; q++) hs[q] = b->mem[q]->sc;
-    if (hipMalloc((void **)&b->dscs, nm * sizeof(Scal *)) != hipSuccess ||
-        hipMalloc((void **)&b->dflags, nm * sizeof(unsigned int)) != hipSuccess)
+    if (hipMallocAsync((void **)&b->dscs, nm * sizeof(Scal *), b->stream) != hipSuccess ||
+        hipMallocAsync((void **)&b->dflags, nm * sizeof(unsigned int), b->stream) != hipSuccess)
       return GK_ERR_NOMEM;
-    if (hipHostMalloc((void **)&b->hflags, nm * sizeof(unsigned int), hipHostMallocDefault) !=
-        hipSuccess) {
-      b->hflags = nullptr;
-      return GK_ERR_NOMEM;
-    }
+    if (!(b->hflags = (unsigned int *)pinned_get(nm * sizeof(unsigned int)))) return GK_ERR_NOMEM;
     memset(b->hflags, 0, nm * sizeof(unsigned int));
-    HIPCHK(hipMemcpy(b->dscs, hs.data(), nm * sizeof(Scal *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->dscs, hs.data(), nm * sizeof(Scal *), hipMemcpyHostToDevice,
+                          b->stream));
     b->nflags = nm;
   }
+  HIPCHK(hipStreamSynchronize(b->stream));  // the host copies of the tables go out of scope
   b->phase = 0;
   b->dirty = false;
   return GK_OK;
@@ -4089,7 +4331,7 @@ static int batch_build(gk_batch *b) {
 // launches of one batch step at phase p
 static void batch_launch_step(gk_batch *b, int p, bool timed) {
   for (int k = 0; k < SK_COUNT; k++)
-    for (int gp = 0; gp < 2; gp++) {
+    for (int gp = 0; gp < NGROUP; gp++) {
       const int nw = b->nwork[gp][k];
       if (!nw) continue;
       const int nblocks = (nw + NXCD - 1) / NXCD * NXCD;  // one block per entry (xcd_item)
@@ -4101,16 +4343,19 @@ static void batch_launch_step(gk_batch *b, int p, bool timed) {
       }
       const void *args = b->dargs + b->args_off[gp][p][k];
       const WorkItem *work = b->dwork + b->work_off[gp][k];
-      if (gp == 1)
-        launch_batch<float>(k, args, work, nw, nblocks, b->stream);
-      else
-        launch_batch<double>(k, args, work, nw, nblocks, b->stream);
+      switch (gp) {
+        case 0: launch_batch<double, false>(k, args, work, nw, nblocks, b->stream); break;
+        case 1: launch_batch<float, false>(k, args, work, nw, nblocks, b->stream); break;
+        case 2: launch_batch<double, true>(k, args, work, nw, nblocks, b->stream); break;
+        default: launch_batch<float, true>(k, args, work, nw, nblocks, b->stream); break;
+      }
       if (timed) {
         (void)hipEventRecord(e1, b->stream);
         (void)hipEventSynchronize(e1);
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        std::string nm = std::string(sk_name[k]) + (gp ? "/f32" : "/f64");
+        std::string nm = std::string(sk_name[k]) + ((gp & 1) ? "/f32" : "/f64") +
+                         ((gp & 2) ? "/b0" : "");
         int slot = -1;
         for (size_t q = 0; q < b->names.size(); q++)
           if (b->names[q] == nm) slot = (int)q;

@@ -375,68 +375,69 @@ static void b0_full(void) {
       }
 }
 
-/* Device representation of B0: the device path supports a spatially uniform
-   background (the per-face/edge averages then equal the representative
-   values to roundoff).  Evaluated at the first active location and checked
-   against a deterministic sample of other active locations. */
+/* Device representation of B0.  The reference's arrays (face and edge
+   quadratures, cell values, the Lorentz force of the background stress) are
+   computed in full whenever a background field is set.  When every array
+   holds one value, bit for bit, over the points the device reads (the
+   active cells and the faces and edges around them), the device takes those
+   values as constants (mode 1); otherwise it takes the arrays (mode 2): a
+   field that varies in space, or a uniform one whose quadratures differ in
+   the last bits from face to face (grid widths that are not powers of two),
+   which the device must see exactly as the reference does. */
 gk_b0_info gk_b0;
 
-static int b0_close(double a, double b, double scale) {
-  return fabs(a - b) <= 1e-12 * (scale > 1e-300 ? scale : 1.0);
+/* one value over the box, bit for bit (then *val is it) */
+static int same_over(int f, int i0, int i1, int j0, int j1, int k0, int k1, double *val) {
+  const double v = gem[f][i0][j0][k0];
+  for (int i = i0; i <= i1; i++)
+    for (int j = j0; j <= j1; j++)
+      for (int k = k0; k <= k1; k++)
+        if (memcmp(&gem[f][i][j][k], &v, sizeof v) != 0) return 0;
+  *val = v;
+  return 1;
 }
 
-static int b0_uniform_summary(void) {
-  memset(&gk_b0, 0, sizeof gk_b0);
-  if (!gk_b0_active()) return 0;
-  int samples[9][3] = {{is, js, ks}, {ie, je, ke}, {is, je, ks}, {ie, js, ke}, {(is + ie) / 2, (js + je) / 2, (ks + ke) / 2},
-                       {is + 1, js + 2, ks + 3}, {ie - 1, js + 1, ke - 2}, {is + 2, je - 1, ks}, {(is + ie) / 3, je, (ks + ke) / 3}};
-  double ref_cell[3], ref_face[3][8], ref_edge[3][3];
-  double scale = 0.0;
-  for (int s = 0; s < 9; s++) {
-    int i = samples[s][0], j = samples[s][1], k = samples[s][2];
-    double cell[3] = {B10(x1ctr[i][j][k], x2ctr[i][j][k], x3ctr[i][j][k]),
-                      B20(x1ctr[i][j][k], x2ctr[i][j][k], x3ctr[i][j][k]),
-                      B30(x1ctr[i][j][k], x2ctr[i][j][k], x3ctr[i][j][k])};
-    double face[3][8], edge[3][3];
-    for (int d = 0; d < 3; d++) {
-      face_b0_at(d, i, j, k, face[d]);
-      edge_b0_at(d, i, j, k, edge[d]);
-    }
-    if (s == 0) {
-      memcpy(ref_cell, cell, sizeof cell);
-      memcpy(ref_face, face, sizeof face);
-      memcpy(ref_edge, edge, sizeof edge);
-      for (int m = 0; m < 3; m++) scale = fmax(scale, fabs(cell[m]));
-      continue;
-    }
-    for (int m = 0; m < 3; m++) {
-      if (!b0_close(cell[m], ref_cell[m], scale)) return -1;
-      for (int d = 0; d < 3; d++)
-        if (!b0_close(edge[d][m], ref_edge[d][m], scale)) return -1;
-    }
-    for (int d = 0; d < 3; d++)
-      for (int q = 0; q < 8; q++)
-        if (!b0_close(face[d][q], ref_face[d][q], q >= 4 ? scale * scale : scale)) return -1;
+static int b0_constants(void) {
+  int ok = 1;
+  for (int m = 0; m < 3; m++) {
+    ok &= same_over(B0_b1 + m, is, ie, js, je, ks, ke, &gk_b0.cell[m]);
+    ok &= same_over(dmagsf_G_x1dir + m, is, ie, js, je, ks, ke, &gk_b0.lorentz[m]);
   }
-  if (!(Source1_B0 == Field_Default && Source2_B0 == Field_Default && Source3_B0 == Field_Default))
-    return -1;
-  gk_b0.mode = 1;
-  memcpy(gk_b0.cell, ref_cell, sizeof ref_cell);
-  memcpy(gk_b0.face, ref_face, sizeof ref_face);
-  memcpy(gk_b0.edge, ref_edge, sizeof ref_edge);
-  /* the Lorentz force of a uniform field is zero (its stress divergence
-     vanishes and there are no source functions) */
-  return 0;
+  for (int d = 0; d < 3; d++) {
+    /* faces of direction d: [s, e+1] along d, the active cells across; the
+       device reads b1 b2 b3 bn (bsq, bn1..bn3 feed only dmagsf_G) */
+    const int fa = d == 0, fb = d == 1, fc = d == 2;
+    for (int q = 0; q < 8; q++) {
+      double v = gem[B0AfaceI_b1 + 8 * d + q][is][js][ks];
+      if (q < 4) ok &= same_over(B0AfaceI_b1 + 8 * d + q, is, ie + fa, js, je + fb, ks, ke + fc, &v);
+      gk_b0.face[d][q] = v;
+    }
+    /* edges of direction d: [s, e] along d, [s, e+1] across; the device
+       reads the two components across the edge */
+    for (int q = 0; q < 3; q++) {
+      double v = gem[B0AedgeI_b1 + 3 * d + q][is][js][ks];
+      if (q != d)
+        ok &= same_over(B0AedgeI_b1 + 3 * d + q, is, ie + !fa, js, je + !fb, ks, ke + !fc, &v);
+      gk_b0.edge[d][q] = v;
+    }
+  }
+  return ok;
 }
 
-int gk_b0_full_arrays = 0; /* set to fill the full reference arrays too */
+int gk_b0_full_arrays = 1; /* kept for the bindings; the arrays are always computed */
 
 void set_background_field() {
   log_info("Initializing Background Fields...");
-  if (gk_b0_full_arrays && gk_b0_active()) b0_full();
-  if (b0_uniform_summary() != 0) {
-    gk_b0.mode = -1;
-    log_error("background field is not spatially uniform; the device path supports none or uniform B0");
+  memset(&gk_b0, 0, sizeof gk_b0);
+  if (!gk_b0_active()) return; /* mode 0: no background field */
+  b0_full();
+  gk_b0.mode = b0_constants() ? 1 : 2;
+  if (gk_b0.mode == 2) {
+    memset(gk_b0.cell, 0, sizeof gk_b0.cell);
+    memset(gk_b0.face, 0, sizeof gk_b0.face);
+    memset(gk_b0.edge, 0, sizeof gk_b0.edge);
+    memset(gk_b0.lorentz, 0, sizeof gk_b0.lorentz);
+    log_info("background field varies over the grid: the device reads it per point");
   }
 }
 

@@ -29,10 +29,11 @@ extern int gk_bc_kind[2][6];
 extern int gk_bc_record_only;
 
 typedef struct {
-  int mode; /* 0 none, 1 uniform, -1 unsupported */
+  int mode; /* 0 none, 1 one value per array (constants below), 2 per point */
   double cell[3];
   double face[3][8];
   double edge[3][3];
+  double lorentz[3];
 } gk_b0_info;
 extern gk_b0_info gk_b0;
 extern int gk_b0_full_arrays;

@@ -306,7 +306,7 @@ typedef struct {
 } gk_blob_head;
 #define GK_BLOB_MAGIC 0x6b426c6fu
 
-static size_t blob_layout(const gk_blob_head *h, size_t off[GK_G_COUNT + 3 + 1 + 1]) {
+static size_t blob_layout(const gk_blob_head *h, size_t off[GK_G_COUNT + 3 + 1 + 1 + 1]) {
   size_t n = sizeof(gk_blob_head);
   n = (n + 7) / 8 * 8;
   size_t q = 0;
@@ -326,6 +326,9 @@ static size_t blob_layout(const gk_blob_head *h, size_t off[GK_G_COUNT + 3 + 1 +
     const int tb = a == 0 ? 1 : 0, tc = a == 2 ? 1 : 2;
     n += sizeof(int) * (size_t)(h->N[tb] + h->N[tc]);
   }
+  n = (n + 7) / 8 * 8;
+  off[q++] = n; /* per-point background field (b0_mode 2) */
+  if (h->b0_mode == 2) n += sizeof(double) * GK_B0_FIELDS * (size_t)h->N[0] * h->N[1] * h->N[2];
   return n;
 }
 
@@ -344,7 +347,7 @@ long gkh_desc_blob(void *buf, size_t cap) {
   gk_blob_head h;
   memset(&h, 0, sizeof h);
   h.magic = GK_BLOB_MAGIC;
-  h.version = 3;
+  h.version = 4;
   h.ring = d.ring;
   memcpy(h.ring_pole, d.ring_pole, sizeof h.ring_pole);
   h.ring_rows = d.ring_rows;
@@ -367,7 +370,7 @@ long gkh_desc_blob(void *buf, size_t cap) {
   h.rho_floor = d.rho_floor;
   h.p_floor = d.p_floor;
   memcpy(h.gravity, d.gravity, sizeof h.gravity);
-  size_t off[GK_G_COUNT + 5];
+  size_t off[GK_G_COUNT + 6];
   const size_t need = blob_layout(&h, off);
   if (buf && cap >= need) {
     char *b = (char *)buf;
@@ -399,6 +402,11 @@ long gkh_desc_blob(void *buf, size_t cap) {
       memcpy(cm, d.cls[a][1], sizeof(int) * h.N[tc]);
       cm += sizeof(int) * h.N[tc];
     }
+    if (h.b0_mode == 2) {
+      const size_t n3 = (size_t)h.N[0] * h.N[1] * h.N[2];
+      for (int f = 0; f < GK_B0_FIELDS; f++)
+        memcpy(b + off[q] + sizeof(double) * n3 * f, d.b0_field[f], sizeof(double) * n3);
+    }
   }
   gk_tables_free(&t);
   return (long)need;
@@ -412,8 +420,8 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   *out = NULL;
   gk_blob_head h;
   memcpy(&h, blob, sizeof h);
-  if (h.magic != GK_BLOB_MAGIC || h.version != 3) return GK_ERR_ARG;
-  size_t off[GK_G_COUNT + 5];
+  if (h.magic != GK_BLOB_MAGIC || h.version != 4) return GK_ERR_ARG;
+  size_t off[GK_G_COUNT + 6];
   if (blob_layout(&h, off) > n) return GK_ERR_ARG;
   const char *b = (const char *)blob;
   gk_grid_desc d;
@@ -451,6 +459,11 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   }
   memcpy(d.ncls, h.ncls, sizeof d.ncls);
   d.b0_mode = h.b0_mode;
+  if (h.b0_mode == 2) {
+    const size_t n3 = (size_t)h.N[0] * h.N[1] * h.N[2];
+    for (int f = 0; f < GK_B0_FIELDS; f++)
+      d.b0_field[f] = (const double *)(b + off[q] + sizeof(double) * n3 * f);
+  }
   memcpy(d.b0_cell, h.b0_cell, sizeof d.b0_cell);
   memcpy(d.b0_face, h.b0_face, sizeof d.b0_face);
   memcpy(d.b0_edge, h.b0_edge, sizeof d.b0_edge);
@@ -702,9 +715,10 @@ int gkh_main(const char *config_file, int nt) {
   A1 = A2 = A3 = Field_Default;
   set_problem_config();
   /* coordinate system of the seeded problem: spherical for the dipole
-     shell (kind 2), the full sphere (kind 4) and the polar cap (kind 5),
-     Cartesian for the others */
-  gk_coord_type = (gk_problem_kind == 2 || gk_problem_kind == 4 || gk_problem_kind == 5) ? 2 : 0;
+     shells (kinds 2, 11), the full sphere (kind 4) and the polar cap
+     (kind 5), Cartesian for the others */
+  gk_coord_type = (gk_problem_kind == 2 || gk_problem_kind == 4 || gk_problem_kind == 5 ||
+                   gk_problem_kind == 11) ? 2 : 0;
   if (initialize_config(0, 1, config_file) != 0) return -1;
   if (setup_comm_and_types(0) != 0) return -1;
   if (initialize_logging(BASE_LOG_NAME, 0) != 0) return -1;
@@ -750,7 +764,7 @@ int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int fu
   if (initialize_solver(BASE_DAT_NAME) != 0) return -2;
   getweights();
   problem_init();
-  gk_b0_full_arrays = full_b0;
+  (void)full_b0; /* the background-field arrays are always computed */
   set_background_field();
   if (gk_b0.mode < 0) return -3;
   if (doRingAverage == 1) {

@@ -247,6 +247,16 @@ int gk_fill_desc(gk_grid_desc *d, const gk_tables *t, int precision) {
   memcpy(d->b0_cell, gk_b0.cell, sizeof d->b0_cell);
   memcpy(d->b0_face, gk_b0.face, sizeof d->b0_face);
   memcpy(d->b0_edge, gk_b0.edge, sizeof d->b0_edge);
+  memcpy(d->b0_lorentz, gk_b0.lorentz, sizeof d->b0_lorentz);
+  if (gk_b0.mode == 2) { /* the reference's arrays, in the order of gamera_op.h */
+    int q = 0;
+    for (int f = B0AfaceI_b1; f <= B0AfaceK_bn3; f++)
+      if ((f - B0AfaceI_b1) % 8 < 4) d->b0_field[q++] = &gem[f][0][0][0];
+    const int edge[6] = {B0AedgeI_b2, B0AedgeI_b3, B0AedgeJ_b3, B0AedgeJ_b1, B0AedgeK_b1, B0AedgeK_b2};
+    for (int e = 0; e < 6; e++) d->b0_field[q++] = &gem[edge[e]][0][0][0];
+    for (int m = 0; m < 3; m++) d->b0_field[q++] = &gem[B0_b1 + m][0][0][0];
+    for (int m = 0; m < 3; m++) d->b0_field[q++] = &gem[dmagsf_G_x1dir + m][0][0][0];
+  }
   /* the problem's floors and gravity (globals a plugin may set) */
   d->rho_floor = rho_floor;
   d->p_floor = p_floor;

@@ -63,6 +63,18 @@
  *                      gravity of 0.4 in a seeded direction (G01..G03), the
  *                      globals the reference leaves to problem files
  *                      (src/config.c:89-97).
+ *   GK_VARYING_B0_BOX  GK_SINUSOID_SPHERE with a background field that
+ *                      varies in space: B0 = b + 0.3 (sin(2 pi y + q1),
+ *                      sin(2 pi z + q2), sin(2 pi x + q3)), b the seeded unit
+ *                      vector of GK_SINUSOID_BLAST (divergence-free and
+ *                      periodic; its stress divergence, the reference's
+ *                      dmagsf_G, is not zero).
+ *   GK_DIPOLE_B0_SHELL GK_DIPOLE_SHELL's grid, boundaries and state with the
+ *                      dipole as the background field B0 (spherical
+ *                      components B_r = 2 m cos(theta)/r^3,
+ *                      B_theta = m sin(theta)/r^3) and a tenth of it as the
+ *                      evolved field (edge potential A_phi = 0.1 m
+ *                      sin(theta)/r^2).
  *
  * Every random draw comes from a counter-based hash of (seed, stream, global
  * i, j, k), so the state does not depend on the domain decomposition.
@@ -92,6 +104,8 @@ enum {
   GK_INFLOW_BOX = 7,
   GK_ENTROPY_BOX = 8,
   GK_FLOOR_BOX = 9,
+  GK_VARYING_B0_BOX = 10,
+  GK_DIPOLE_B0_SHELL = 11,
 };
 
 
@@ -137,6 +151,13 @@ static double p_A1, p_A2, p_Bamp, p_ph[4];
 static double p_b0[3];
 static double p_ctr[3];
 static double p_m;
+static double p_q[3];  /* phases of the varying background field */
+
+/* the spherical-shell problems (log-spaced radii, symmetric r and theta
+   boundaries) */
+static int gk_shell(void) {
+  return gk_problem_kind == GK_DIPOLE_SHELL || gk_problem_kind == GK_DIPOLE_B0_SHELL;
+}
 
 static void gk_resolve_selection(void) {
   if (gk_problem_kind < 0) {
@@ -163,7 +184,8 @@ static void gk_draw_parameters(void) {
   p_b0[0] = sz * cos(az);
   p_b0[1] = sz * sin(az);
   p_b0[2] = cz;
-  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_polar()) {
+  for (int n = 0; n < 3; n++) p_q[n] = twopi * gk_scalar(22 + n);
+  if (gk_shell() || gk_polar()) {
     p_m = 0.5 + 1.5 * gk_scalar(13);
     p_ctr[0] = 3.0 + 4.0 * gk_scalar(14);               /* r */
     if (gk_polar())
@@ -185,6 +207,33 @@ static double gk_Az_sinusoid(double x, double y, double z) {
 static double gk_Aphi_dipole(double r, double th, double ph) {
   (void)ph;
   return p_m * sin(th) / (r * r);
+}
+
+static double gk_Aphi_dipole_tenth(double r, double th, double ph) {
+  (void)ph;
+  return 0.1 * p_m * sin(th) / (r * r);
+}
+
+/* background fields that vary in space */
+static double gk_B0x_var(double x, double y, double z) {
+  (void)x; (void)z;
+  return p_b0[0] + 0.3 * sin(2.0 * PI * y + p_q[0]);
+}
+static double gk_B0y_var(double x, double y, double z) {
+  (void)x; (void)y;
+  return p_b0[1] + 0.3 * sin(2.0 * PI * z + p_q[1]);
+}
+static double gk_B0z_var(double x, double y, double z) {
+  (void)y; (void)z;
+  return p_b0[2] + 0.3 * sin(2.0 * PI * x + p_q[2]);
+}
+static double gk_B0r_dipole(double r, double th, double ph) {
+  (void)ph;
+  return 2.0 * p_m * cos(th) / (r * r * r);
+}
+static double gk_B0th_dipole(double r, double th, double ph) {
+  (void)ph;
+  return p_m * sin(th) / (r * r * r);
 }
 
 static double gk_B0x(double x, double y, double z) { (void)x; (void)y; (void)z; return p_b0[0]; }
@@ -226,7 +275,7 @@ void set_problem_config() {
     G02 = 0.4 * sz * sin(az);
     G03 = 0.4 * cz;
   }
-  if (gk_problem_kind == GK_DIPOLE_SHELL || gk_polar()) {
+  if (gk_shell() || gk_polar()) {
     x1min_global = 1.0;
     x1max_global = 10.0;
     x2min_global = gk_polar() ? 0.0 : PI / 8.0;
@@ -256,6 +305,16 @@ void set_problem_config() {
     B20 = gk_B0y;
     B30 = gk_B0z;
   }
+  if (gk_problem_kind == GK_VARYING_B0_BOX) {
+    B10 = gk_B0x_var;
+    B20 = gk_B0y_var;
+    B30 = gk_B0z_var;
+  }
+  if (gk_problem_kind == GK_DIPOLE_B0_SHELL) {
+    B10 = gk_B0r_dipole;
+    B20 = gk_B0th_dipole;
+    B30 = Field_Default;
+  }
 }
 
 /*
@@ -265,7 +324,7 @@ void set_problem_config() {
  * the geometry arrays.
  */
 int gk_problem_custom_grid(void) {
-  if (gk_problem_kind != GK_DIPOLE_SHELL && !gk_polar()) return 0;
+  if (!gk_shell() && !gk_polar()) return 0;
   for (int i = isg; i <= ieg + 1; i++) {
     int gi = i - is + proc_coords[0] * config.ni;
     double r = pow(10.0, (double)gi / (double)config.ni_global);
@@ -276,7 +335,7 @@ int gk_problem_custom_grid(void) {
 }
 
 void boundary_conditions() {
-  if (gk_problem_kind == GK_DIPOLE_SHELL) {
+  if (gk_shell()) {
     gas_bc_symmetric_i_low();
     gas_bc_symmetric_i_high();
     gas_bc_symmetric_j_low();
@@ -350,7 +409,7 @@ static void gk_init_sinusoid(int sphere) {
          Field_Default, Field_Default);
 }
 
-static void gk_init_dipole_shell(void) {
+static void gk_init_dipole_shell(int b0) {
   double xc = p_ctr[0] * sin(p_ctr[1]) * cos(p_ctr[2]);
   double yc = p_ctr[0] * sin(p_ctr[1]) * sin(p_ctr[2]);
   double zc = p_ctr[0] * cos(p_ctr[1]);
@@ -372,7 +431,7 @@ static void gk_init_dipole_shell(void) {
       }
     }
   }
-  A3 = gk_Aphi_dipole;
+  A3 = b0 ? gk_Aphi_dipole_tenth : gk_Aphi_dipole;
   AinitB(gem[mag_bi], gem[mag_bj], gem[mag_bk], A1, A2, A3, Field_Default,
          Field_Default, Field_Default);
 }
@@ -474,7 +533,13 @@ void problem_init() {
       gk_init_floor_box();
       break;
     case GK_DIPOLE_SHELL:
-      gk_init_dipole_shell();
+      gk_init_dipole_shell(0);
+      break;
+    case GK_DIPOLE_B0_SHELL:
+      gk_init_dipole_shell(1);
+      break;
+    case GK_VARYING_B0_BOX:
+      gk_init_sinusoid(1);
       break;
     case GK_POLE_SPHERE:
       gk_init_pole_sphere(0);

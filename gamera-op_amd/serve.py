@@ -291,14 +291,18 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     def retire(a, status):
         nonlocal used
         r, S = a[0], a[1]
+        tr = time.perf_counter()
         sc = S.scalars()
+        stats["retire_scalars_s"] += time.perf_counter() - tr
         done_steps = r.steps - max(a[2], 0)
         res = Result(r.rid, r.arrival, a[3], now(), done_steps, r.cells * done_steps,
                      sc["time"], status=status)
         if digests:
             res.digest = state_digest(S)
         results.append(res)
+        tr = time.perf_counter()
         S.close()
+        stats["retire_close_s"] += time.perf_counter() - tr
         used -= a[4]
         if log:
             log(f"{status} rid={r.rid} latency={res.latency:.3f}s active={len(active)}")

@@ -90,7 +90,8 @@ typedef struct {
   const int *cls[3][2];
   int ncls[3][2];
   /* background field B0 (src/solver_func.c:1828-2077): mode 0 none,
-     1 spatially uniform (constants below) */
+     1 the same at every point (constants below), 2 given per point
+     (b0_field) */
   int b0_mode;
   double b0_cell[3];
   double b0_face[3][8];    /* per face direction: b1 b2 b3 bn bsq bn1 bn2 bn3 */
@@ -117,7 +118,16 @@ typedef struct {
      values are rejected (GK_ERR_ARG). */
   double rho_floor, p_floor;
   double gravity[3];
+  /* b0_mode 2: the reference's background-field arrays, GK_B0_FIELDS host
+     arrays in the reference layout (NI*NJ*NK doubles each), in this order:
+     face averages B0AfaceI_b1 b2 b3 bn, B0AfaceJ_b1 b2 b3 bn, B0AfaceK_b1 b2
+     b3 bn (12); edge averages B0AedgeI_b2 b3, B0AedgeJ_b3 b1, B0AedgeK_b1 b2
+     (6); cell-centre B0_b1 b2 b3 (3); Lorentz force dmagsf_G_x1dir x2dir
+     x3dir (3).  Copied to the device by gk_create. */
+  const double *b0_field[24];
 } gk_grid_desc;
+
+#define GK_B0_FIELDS 24
 
 typedef struct gk_solver gk_solver;
 

@@ -45,9 +45,15 @@ STATE_CASES = [
     (8, 32, 16, 16, 8, 10),
     # positivity floors crossed (both branches), non-default floors, gravity
     (9, 31, 24, 20, 12, 20),
+    # uniform background field whose face quadratures differ in the last
+    # bits from face to face (widths 1/32, 1/24, 1/16)
+    (1, 15, 32, 24, 16, 12),
+    # background field varying in space: sinusoidal box, dipole shell
+    (10, 41, 16, 12, 16, 12),
+    (11, 42, 16, 12, 16, 10),
 ]
 GEOM_CASES = [(0, 1, 16, 16, 8), (1, 3, 16, 16, 16), (2, 4, 16, 12, 16), (0, 7, 40, 24, 12),
-              (4, 8, 12, 16, 32)]
+              (4, 8, 12, 16, 32), (10, 41, 16, 12, 16), (11, 42, 16, 12, 16)]
 CONFIG0 = (0, 1, 128, 128, 8, 200)
 
 COORDS = ["x1", "x2", "x3", "x1c", "x2c", "x3c", "x1ctr", "x2ctr", "x3ctr", "dx1", "dx2", "dx3"]

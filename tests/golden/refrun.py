@@ -36,7 +36,7 @@ def ref_available(spherical=False):
 def run_reference(kind, seed, ni, nj, nk, nt, geometry=False, threads=8, workdir=None):
     """Run the reference for nt steps; returns the record dict.  With workdir
     the run happens there and its files (HDF5 dumps, logs) are kept."""
-    spherical = kind in (2, 4, 5)
+    spherical = kind in (2, 4, 5, 11)
     exe = os.path.join(REF_DIR, "ref_driver_sph" if spherical else "ref_driver_cart")
     with tempfile.TemporaryDirectory(prefix="gkref") as tmp:
         d = workdir or tmp

@@ -10,18 +10,11 @@ B_NAMES = ["Bi", "Bj", "Bk"]
 
 def oracle_from_problem(P, fast=False):
     """An oracle loaded with the problem's geometry, weights, B0 arrays and
-    initial primitive state (before the prologue).  A problem with a
-    background field must have been set up with its full B0 arrays
-    (Problem(..., full_b0=True)): the oracle reads the field at every cell,
-    face and edge, and the set-up without them fills only the constants the
-    device takes."""
-    if P.b0_mode() != 0 and not P.full_b0:
-        raise ValueError("oracle_from_problem needs Problem(..., full_b0=True) for a problem "
-                         "with a background field")
+    initial primitive state (before the prologue)."""
     ni, nj, nk = P.n
     bc = P.bc_table()
     kinds = []
-    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5, 6, 7) else [False, False, True]
+    periodic = [P.coord == 0, P.coord == 0, True] if P.kind not in (2, 4, 5, 6, 7, 11) else [False, False, True]
     # recorded gas kinds (host) -> oracle kinds: symmetric, reflective,
     # fixed, extrapolated, pole; anything else leaves the ghosts as they are
     to_oracle = {1: 1, 2: 4, 3: 6, 4: 5, 6: 3}

@@ -25,6 +25,10 @@ MIX = [
     # pole ring-averaged grids (full sphere FP64, polar cap FP32-bulk)
     (go.POLE_SPHERE, 18, 8, 12, 32, 0),
     (go.POLE_CAP, 19, 8, 10, 64, 1),
+    # background fields read per point, in both forms (their own launch
+    # groups)
+    (go.VARYING_B0_BOX, 20, 16, 12, 16, 1),
+    (go.DIPOLE_B0_SHELL, 21, 12, 8, 12, 0),
 ]
 
 

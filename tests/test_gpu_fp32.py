@@ -31,6 +31,9 @@ CASES = [
     # reflective walls in x, extrapolated boundaries in y
     (go.WALL_BOX, 15, 24, 24, 16, 40),
     (go.INFLOW_BOX, 24, 24, 16, 16, 30),
+    # background field read per point
+    (go.VARYING_B0_BOX, 44, 24, 24, 16, 30),
+    (go.DIPOLE_B0_SHELL, 45, 24, 16, 24, 30),
 ]
 
 

@@ -32,6 +32,14 @@ CASES = [
     # fixed boundary at x = 0, extrapolated at x = 1, symmetric in y
     (go.INFLOW_BOX, 22, 16, 12, 8, 12),
     (go.INFLOW_BOX, 23, 12, 16, 12, 8),
+    # background field read per point: a uniform field whose face
+    # quadratures differ in the last bits (widths 1/32, 1/24, 1/16), fields
+    # varying in space (a sinusoidal box, a dipole shell), and the staged k
+    # tiles with their leftover columns in that form
+    (go.SINUSOID_BLAST, 15, 32, 24, 16, 12),
+    (go.VARYING_B0_BOX, 41, 16, 12, 16, 12),
+    (go.DIPOLE_B0_SHELL, 42, 16, 12, 16, 10),
+    (go.VARYING_B0_BOX, 43, 12, 8, 66, 8),
 ]
 
 

@@ -90,17 +90,23 @@ def test_boundary_kinds_recorded_from_plugin():
         P.close()
 
 
-def test_uniform_background_summarised_for_device():
-    P = go.Problem(go.SINUSOID_BLAST, 3, 12, 12, 12)
-    try:
-        assert P.b0_mode() == 1
-    finally:
-        P.close()
-    P = go.Problem(go.SINUSOID, 3, 12, 12, 12)
-    try:
-        assert P.b0_mode() == 0
-    finally:
-        P.close()
+def test_background_field_mode_for_device():
+    """The device takes B0 as constants only where the reference's arrays
+    hold one value bit for bit over the points it reads (mode 1), and per
+    point otherwise (mode 2): a uniform field on widths 1/16 is the first,
+    on widths 1/12 (face quadratures that differ in the last bits) or a
+    field varying in space the second; no field is mode 0."""
+    for kind, n, mode in [(go.SINUSOID_BLAST, (16, 16, 16), 1),
+                          (go.SINUSOID_BLAST, (12, 12, 12), 2),
+                          (go.SINUSOID_BLAST, (32, 24, 16), 2),
+                          (go.VARYING_B0_BOX, (16, 16, 16), 2),
+                          (go.DIPOLE_B0_SHELL, (16, 12, 16), 2),
+                          (go.SINUSOID, (12, 12, 12), 0)]:
+        P = go.Problem(kind, 3, *n)
+        try:
+            assert P.b0_mode() == mode, (kind, n)
+        finally:
+            P.close()
 
 
 def test_read_config_yaml_schema():

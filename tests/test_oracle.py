@@ -32,7 +32,12 @@ STATE_CASES = [(0, 1, 16, 16, 8, 10), (0, 2, 24, 20, 12, 30), (1, 3, 16, 16, 16,
                # entropy channel carried (p_S = p at the start)
                (8, 32, 16, 16, 8, 10),
                # positivity floors crossed, non-default floors, gravity
-               (9, 31, 24, 20, 12, 20)]
+               (9, 31, 24, 20, 12, 20),
+               # uniform background field whose face quadratures differ in
+               # the last bits from face to face
+               (1, 15, 32, 24, 16, 12),
+               # background field varying in space: sinusoidal box, dipole shell
+               (10, 41, 16, 12, 16, 12), (11, 42, 16, 12, 16, 10)]
 
 
 def sha(a):
@@ -55,8 +60,10 @@ def test_oracle_reproduces_reference_bitwise(kind, seed, ni, nj, nk, steps):
         assert O.scalar("time_sim") == sc[2]
         # the reference max-updates divB_max inside an OpenMP loop without a
         # reduction (src/solver_func.c:324-343), so its value can miss the
-        # true maximum but never exceed it
-        assert sc[3] <= O.scalar("divB_max") < 1e-12
+        # true maximum but never exceed it; div B carries the background
+        # field's face normals, whose quadratures of a dipole (|B0| up to 4)
+        # leave a few 1e-12
+        assert sc[3] <= O.scalar("divB_max") < 1e-10
     finally:
         P.close()
 
