@@ -2614,6 +2614,7 @@ struct gk_solver {
   Geo geo;
   B0c b0;
   int device;          // HIP device the solver was created on
+  bool own_stream;     // the stream came from the pool (else the caller's)
   double *b0a;         // per-point background field (b0_mode 2), else null
   int spherical;
   int bc[2][6];
@@ -2867,7 +2868,10 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   keep_pool(s->device);
   // every upload of the set-up runs on the solver's own stream, so creating
   // a solver does not wait for work other solvers or batches have in flight
-  if (stream_get(s->device, &s->stream) != hipSuccess) {
+  s->own_stream = d->stream == nullptr;
+  if (!s->own_stream)
+    s->stream = (hipStream_t)d->stream;
+  else if (stream_get(s->device, &s->stream) != hipSuccess) {
     delete s;
     return GK_ERR_DEVICE;
   }
@@ -3074,7 +3078,7 @@ extern "C" int gk_destroy(gk_solver *s) {
   pinned_put(s->pin);
   pinned_put(s->ctl_host);
   for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
-  stream_put(s->device, s->stream);
+  if (s->own_stream) stream_put(s->device, s->stream);
   delete s;
   return GK_OK;
 }
@@ -4210,6 +4214,8 @@ static int launch_group(const gk_solver *s) {
 struct gk_batch {
   std::vector<gk_solver *> mem;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // members' set-up and read-back (gk_batch_side_stream)
+  int device = 0;
   bool dirty = true;
   int phase = 0;  // batch steps since the tables were built, mod 6
   char *dargs = nullptr;
@@ -4390,11 +4396,26 @@ static int batch_capture(gk_batch *b, int first_phase, int nsteps, hipGraphExec_
 extern "C" int gk_batch_create(gk_batch **out) {
   if (!out) return GK_ERR_ARG;
   gk_batch *b = new gk_batch();
+  // the batch's stream and its side stream are created together, so that
+  // the runtime maps them to different hardware queues: work on the side
+  // stream then never waits behind the batch's steps
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return GK_ERR_DEVICE;
   }
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return GK_ERR_DEVICE;
+  }
+  (void)hipGetDevice(&b->device);
   *out = b;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_side_stream(gk_batch *b, void **stream) {
+  if (!b || !stream) return GK_ERR_ARG;
+  *stream = (void *)b->side;
   return GK_OK;
 }
 
@@ -4404,6 +4425,10 @@ extern "C" int gk_batch_destroy(gk_batch *b) {
   for (gk_solver *s : b->mem) s->batch = nullptr;
   batch_free_tables(b);
   (void)hipStreamDestroy(b->stream);
+  // solvers created on the side stream may outlive the batch: the stream
+  // stays alive in the pool
+  (void)hipStreamSynchronize(b->side);
+  stream_put(b->device, b->side);
   delete b;
   return GK_OK;
 }

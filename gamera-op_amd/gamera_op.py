@@ -82,7 +82,7 @@ def _load():
     h.gkh_solver_from_blob.restype = ctypes.c_int
     h.gkh_solver_from_blob.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_void_p),
-                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_void_p)]
     h.gkh_main.restype = ctypes.c_int
     h.gkh_main.argtypes = [ctypes.c_char_p, ctypes.c_int]
@@ -108,6 +108,7 @@ def _load():
                        ("gk_batch_step_async", [vp, ctypes.c_int]),
                        ("gk_batch_sync", [vp]),
                        ("gk_batch_set_graphs", [vp, ctypes.c_int]),
+                       ("gk_batch_side_stream", [vp, ctypes.POINTER(vp)]),
                        ("gk_batch_enable_timing", [vp, ctypes.c_int]),
                        ("gk_batch_kernel_times", [vp, ctypes.POINTER(ctypes.c_char_p),
                                                   ctypes.POINTER(ctypes.c_double),
@@ -347,7 +348,7 @@ class Solver:
         self.precision = precision
 
     @classmethod
-    def from_parts(cls, blob, prim, bface, precision=0):
+    def from_parts(cls, blob, prim, bface, precision=0, stream=None):
         """Solver from a descriptor blob (Problem.desc_blob) and an initial
         state: 6 primitive and 3 face-B arrays in the reference layout."""
         self = cls.__new__(cls)
@@ -359,8 +360,8 @@ class Solver:
         P = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in prim])
         B = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in bface])
         p = ctypes.c_void_p()
-        _check(self.h.gkh_solver_from_blob(blob, len(blob), precision, P, B, ctypes.byref(p)),
-               "gkh_solver_from_blob")
+        _check(self.h.gkh_solver_from_blob(blob, len(blob), precision, P, B, stream,
+                                           ctypes.byref(p)), "gkh_solver_from_blob")
         self.handle = p
         self.shape = shape
         self.n = tuple(x - 9 for x in shape)
@@ -479,6 +480,13 @@ class Batch:
 
     def enable_timing(self, on=True):
         _check(self.op.gk_batch_enable_timing(self.handle, 1 if on else 0), "gk_batch_enable_timing")
+
+    def side_stream(self):
+        """The batch's side stream (gk_batch_side_stream) as a pointer value,
+        for solvers whose set-up and read-back should run beside its steps."""
+        st = ctypes.c_void_p()
+        _check(self.op.gk_batch_side_stream(self.handle, ctypes.byref(st)), "gk_batch_side_stream")
+        return st
 
     def kernel_times(self):
         n = ctypes.c_int(64)

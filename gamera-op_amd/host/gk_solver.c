@@ -413,9 +413,10 @@ long gkh_desc_blob(void *buf, size_t cap) {
 }
 
 /* Solver from a descriptor blob and an initial state (reference layout,
-   current time level only); prologue done, time set to 0. */
+   current time level only) on the given HIP stream (NULL: its own);
+   prologue done, time set to 0. */
 int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double *const prim[6],
-                         const double *const bface[3], gk_solver **out) {
+                         const double *const bface[3], void *stream, gk_solver **out) {
   if (!blob || !out || n < sizeof(gk_blob_head)) return GK_ERR_ARG;
   *out = NULL;
   gk_blob_head h;
@@ -476,6 +477,7 @@ int gkh_solver_from_blob(const void *blob, size_t n, int precision, const double
   d.ring_rows = h.ring_rows;
   memcpy(d.ring_nchunk, h.ring_nchunk, sizeof d.ring_nchunk);
   gk_solver *s = NULL;
+  d.stream = stream; /* NULL: the solver's own stream */
   int rc = gk_create(&d, &s);
   if (rc != GK_OK) return rc;
   rc = gk_set_state(s, prim, bface, NULL, NULL);

@@ -206,7 +206,7 @@ class Preparer:
         return self.pool.apply_async(_worker_prepare, (r.kind, r.seed, r.shape))
 
     @staticmethod
-    def solver(result, precision, poison=False):
+    def solver(result, precision, poison=False, stream=None):
         name, blob, shape = result
         shm = shared_memory.SharedMemory(name=name)
         try:
@@ -215,7 +215,7 @@ class Preparer:
                     for q in range(9)]
             if poison:
                 arrs[0][POISON_CELL] = np.nan
-            S = go.Solver.from_parts(blob, arrs[:6], arrs[6:], precision)
+            S = go.Solver.from_parts(blob, arrs[:6], arrs[6:], precision, stream)
             del arrs
         finally:
             shm.close()
@@ -261,6 +261,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     setup = _Setup() if preparer is None else None
     ahead = lookahead or (2 * preparer.workers if preparer is not None else 0)
     B = go.Batch()
+    side = B.side_stream()
     queue = collections.deque()      # arrived, not yet being prepared
     preparing = collections.deque()  # (request, async result) in arrival order
     active = []  # [request, solver, steps_left, admitted_time, bytes]
@@ -327,7 +328,9 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
                         break
                     _, res = preparing.popleft()
                     ta = time.perf_counter()
-                    admit(r, Preparer.solver(res.get(), r.precision, r.poison))
+                    # set-up on the batch's side stream: it runs beside the
+                    # quantum in flight instead of waiting behind it
+                    admit(r, Preparer.solver(res.get(), r.precision, r.poison, side))
                     stats["admit_s"] += time.perf_counter() - ta
             else:
                 # host set-up in this process: one request per pass while the

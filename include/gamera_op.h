@@ -125,6 +125,11 @@ typedef struct {
      (6); cell-centre B0_b1 b2 b3 (3); Lorentz force dmagsf_G_x1dir x2dir
      x3dir (3).  Copied to the device by gk_create. */
   const double *b0_field[24];
+  /* HIP stream (a hipStream_t) for the solver's own work: uploads, steps,
+     read-backs.  NULL: the solver takes a stream of its own.  A given stream
+     must outlive the solver; gk_batch_side_stream provides one whose work
+     never waits behind a batch's steps. */
+  void *stream;
 } gk_grid_desc;
 
 #define GK_B0_FIELDS 24
@@ -250,6 +255,11 @@ long long gk_batch_budget(gk_batch *b, gk_solver *s);
    non-finite value, GK_OK if not, GK_ERR_ARG if it is not a member */
 int gk_batch_status(gk_batch *b, gk_solver *s);
 int gk_batch_set_graphs(gk_batch *b, int on); /* default on */
+/* A second stream of the batch, created with its own so that the two map to
+   different hardware queues: solvers created on it (gk_grid_desc.stream) can
+   be set up, read back and destroyed while the batch steps, without waiting
+   for its launched steps.  It stays valid after gk_batch_destroy. */
+int gk_batch_side_stream(gk_batch *b, void **stream);
 int gk_batch_enable_timing(gk_batch *b, int on);
 int gk_batch_kernel_times(gk_batch *b, const char **names, double *ms, int *n);
 
