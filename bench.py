@@ -170,6 +170,69 @@ def time_sphere(kind, dims, seed, warmup, steps, precision=0):
     return el, kt, sc
 
 
+def run_solve_path(args):
+    """The facade's solve(nt) on the reference's CPU-runnable case (128x128x8
+    sinusoid, FP64, 200 steps) against gk_step(nt) on the same device state:
+    solve() runs the reference's loop with time_stop and the status-line
+    cadence checked on the device, ending a run of steps only where one of
+    them falls due.  Two configurations: time_stop finite beyond the run's
+    end with no status cadence, and the same with a status line every tenth
+    of an output interval equal to the run's length (about 11 host round
+    trips).  solve() also reads both time levels back at the end; that
+    read-back is timed on its own and reported beside the totals."""
+    sys.path.insert(0, PKG)
+    import ctypes
+    import gamera_op as go
+    nt = 200
+    P = go.Problem(go.SINUSOID, args.seed, 128, 128, 8)
+    h = P.h
+    t_stop = ctypes.c_double.in_dll(h, "time_stop")
+    t_out = ctypes.c_double.in_dll(h, "output_interval")
+    log_seq = ctypes.c_int.in_dll(h, "log_seq_num")  # a fresh program starts it at 0
+
+    def fresh():
+        P.reseed(args.seed)
+        P.device_open(0)
+
+    res = {"step": [], "fetch": [], "solve_far": [], "solve_status": []}
+    t_end = None
+    for rep in range(4):  # the first round is warm-up
+        fresh()
+        t0 = time.perf_counter()
+        P.device_step(nt)
+        t1 = time.perf_counter()
+        P.device_fetch()
+        t2 = time.perf_counter()
+        t_end = P.device_scalars()["time"]
+        for key, stop, out in (("solve_far", 2.0 * t_end, 1e300),
+                               ("solve_status", 2.0 * t_end, t_end)):
+            fresh()
+            t_stop.value, t_out.value, log_seq.value = stop, out, 0
+            t3 = time.perf_counter()
+            P.solve(nt)
+            t4 = time.perf_counter()
+            t_stop.value, t_out.value = 1e300, 1e300
+            if P.device_scalars()["time"] != t_end:
+                raise RuntimeError("solve() and gk_step() ended at different times")
+            if rep:
+                res[key].append(t4 - t3)
+        if rep:
+            res["step"].append(t1 - t0)
+            res["fetch"].append(t2 - t1)
+    P.close()
+    t_stop.value, t_out.value = 1e300, 1e300
+    med = {k: sorted(v)[len(v) // 2] * 1e3 for k, v in res.items()}
+    out = {"case": "128x128x8 sinusoid, FP64, 200 steps", "gk_step_ms": med["step"],
+           "readback_ms": med["fetch"], "solve_far_stop_ms": med["solve_far"],
+           "solve_status_lines_ms": med["solve_status"],
+           "ratio_far": med["solve_far"] / (med["step"] + med["fetch"]),
+           "ratio_status": med["solve_status"] / (med["step"] + med["fetch"]),
+           "note": "ratios are solve(200) over gk_step(200) plus the read-back of both "
+                   "time levels that solve() ends with; medians of 3 after a warm-up"}
+    log(f"bench: solve path {out}")
+    return out
+
+
 def run_ring(args):
     """Full sphere (theta in [0, pi], log r in [1, 10], phi periodic) with
     pole ring averaging at both poles, FP64, against the same grid of the
@@ -430,12 +493,13 @@ def main():
     ap.add_argument("--ens-instances", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=300)
     ap.add_argument("--no-ring", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the solve(nt) run-control leg")
     ap.add_argument("--ring-only", action="store_true",
                     help="run only the full-sphere ring-average leg (profiling)")
     ap.add_argument("--ring-dims", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--ring-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["serve", "ensemble"], default=None,
+    ap.add_argument("--only", choices=["serve", "ensemble", "solve"], default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -452,9 +516,11 @@ def main():
     if args.ring_only:
         print(json.dumps({"pole_ring": run_ring(args)}), flush=True)
         return
-    if args.only in ("serve", "ensemble"):  # one leg alone (profiling, tuning)
+    if args.only in ("serve", "ensemble", "solve"):  # one leg alone (profiling, tuning)
         sys.path.insert(0, PKG)
-        if args.only == "serve":
+        if args.only == "solve":
+            out = {"solve_path": run_solve_path(args)}
+        elif args.only == "serve":
             import serve
             preparer = serve.Preparer(args.serve_workers or None)
             out = {"serving": run_serving(args, preparer)}
@@ -486,6 +552,7 @@ def main():
         with open(tfile) as fp:
             traffic = json.load(fp).get("bytes_per_launch")
     ring = None if args.no_ring else run_ring(args)
+    solve_path = None if args.no_solve else run_solve_path(args)
     ens = None if args.no_ensemble else run_ensemble(args)
     srv = None if args.no_serve else run_serving(args, preparer)
     if preparer is not None:
@@ -526,6 +593,7 @@ def main():
         "final_scalars": {"time": sc["time"], "dt": sc["dt"], "divB": sc["divB"]},
         "fp32_bulk": fp32,
         "pole_ring": ring,
+        "solve_path": solve_path,
         "ensemble": ens,
         "serving": srv,
         "cpu_baseline": cpu,

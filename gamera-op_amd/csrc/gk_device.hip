@@ -306,6 +306,23 @@ __device__ __forceinline__ const R *rtab(const Geo &g, int f, const int idx[3]) 
     return (const R *)(g.recwf[A] + off);
 }
 
+// The same table when the grid has one class on each transverse axis of A
+// (Cartesian grids whose cell widths are bitwise equal along each transverse
+// axis, and every spherical grid): then it depends on the face alone, and a
+// march position's face is uniform across a wave.  The 18 values are read
+// through the constant address space (the tables are read-only while a step
+// runs), so a wave fetches them once with scalar loads into scalar registers
+// instead of into 36 (FP64) vector registers of every lane.
+template <int A, typename R>
+__device__ __forceinline__ void utab(const Geo &g, int f, R T[GK_RECW_STRIDE]) {
+  typedef const __attribute__((address_space(4))) R CR;
+  const R *p = (sizeof(R) == 8 ? (const R *)g.recw[A] : (const R *)g.recwf[A]) +
+               (long)f * GK_RECW_STRIDE;
+  CR *c = (CR *)(const void *)p;
+#pragma unroll
+  for (int n = 0; n < GK_RECW_STRIDE; n++) T[n] = c[n];
+}
+
 template <typename R>
 __device__ __forceinline__ R sgn(R x) {
   return (x > R(0)) ? R(1) : ((x < R(0)) ? R(-1) : R(0));
@@ -786,7 +803,7 @@ __device__ __forceinline__ void march_lr(const Window<NF, R> &w, int m, const R 
   Rr = pdm_limit(fr, w.x[m][3], w.x[m][2], w.x[m][1], T[16], T[17]);
 }
 
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, const int by,
                                         const int bz) {
   if (halted(a.sc)) return;
@@ -862,11 +879,17 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
         march_load<9, R, R>(w, P, Q, c + 3 * sd, cc, 7);
         bn = Bd[c];
       }
+      R Tu[UT ? GK_RECW_STRIDE : 1];
+      const R *Tf = T;
+      if constexpr (UT) {
+        utab<D, R>(a.geo, f, Tu);
+        Tf = Tu;
+      }
       R L[6], Rs[6], bl[3], br[3];
 #pragma unroll
-      for (int m = 0; m < 6; m++) march_lr(w, m, T, L[m], Rs[m]);
+      for (int m = 0; m < 6; m++) march_lr(w, m, Tf, L[m], Rs[m]);
 #pragma unroll
-      for (int m = 0; m < 3; m++) march_lr(w, 6 + m, T, bl[m], br[m]);
+      for (int m = 0; m < 3; m++) march_lr(w, 6 + m, Tf, bl[m], br[m]);
       V[c] = R(0.5) * (L[va] + Rs[va]);
       V[fs + c] = R(0.5) * (L[vb] + Rs[vb]);
       R Fo[9], b0[4];
@@ -886,10 +909,16 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
     for (int p = 0; p < 7; p++) march_load<2, R, R>(w, P, Q, c + (p - 4) * sd, cc, p);
     for (int f = f0; f < f1; f++) {
       march_load<2, R, R>(w, P, Q, c + 3 * sd, cc, 7);
+      R Tu[UT ? GK_RECW_STRIDE : 1];
+      const R *Tf = T;
+      if constexpr (UT) {
+        utab<D, R>(a.geo, f, Tu);
+        Tf = Tu;
+      }
       R l, rr;
-      march_lr(w, 0, T, l, rr);
+      march_lr(w, 0, Tf, l, rr);
       V[c] = R(0.5) * (l + rr);
-      march_lr(w, 1, T, l, rr);
+      march_lr(w, 1, Tf, l, rr);
       V[fs + c] = R(0.5) * (l + rr);
       march_shift(w);
       c += sd;
@@ -898,15 +927,15 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
   }
 }
 
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(256) void k_march(SweepArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_march_body<D, R, B0V>(a, bx, by, bz);
+  k_march_body<D, R, B0V, UT>(a, bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
@@ -916,7 +945,7 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_march_body<D, R, B0V>(args[arg], bx, by, bz);
+  k_march_body<D, R, B0V, UT>(args[arg], bx, by, bz);
 }
 
 // k sweep (D = 2) with the stencil staged in LDS.  A block is one wave
@@ -1174,13 +1203,17 @@ struct TableWalk {
   }
 };
 
-// Occupancy bound of the edge-field kernels.  The FP64 E_j and E_k marches
-// keep three windows and reconstruct a fourth field directly; at up to two
-// waves per SIMD they have the registers to issue a position's table and
-// field loads together (at three the compiler loads the table piecewise).
-template <int D, typename R>
+// Occupancy bounds of the edge-field kernels.  The FP64 E_j and E_k marches
+// keep three windows and reconstruct a fourth field directly; at two waves
+// per SIMD they have the registers to issue a position's table and field
+// loads together (at three the compiler loads the table piecewise), and at
+// least two keep the compiler from spending more registers than two waves
+// leave (with uniform tables it would otherwise drop to one).
+template <int D, typename R, bool UT>
 struct EmfWaves {
-  static constexpr int hi = (sizeof(R) == 8 && D != 0) ? 2 : 8;
+  static constexpr bool two = sizeof(R) == 8 && D != 0;
+  static constexpr int lo = two ? 2 : 1;
+  static constexpr int hi = two ? 2 : 8;
 };
 
 // STAGED (E_i and E_j, whose direct reconstructions run along k; the plan
@@ -1190,7 +1223,7 @@ struct EmfWaves {
 // staged in LDS (current and half-step values), and each lane reads its
 // 8-cell stencil from there instead of loading it.  Whole 64-edge tiles only;
 // the unstaged form covers the rest.
-template <int D, typename R, bool STAGED, bool B0V>
+template <int D, typename R, bool STAGED, bool B0V, bool UT>
 __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx, const int by,
                                             const int bz) {
   if (halted(a.sc)) return;
@@ -1251,7 +1284,20 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   // direct reconstructions: v and F1 along A2 (E_i), else F2 along R2
   constexpr int DA = VW ? R2 : A2;
   TableWalk<DA, MA, R> tw;
-  tw.init(a.geo, idx[DA], idx, f0);
+  // uniform tables: the direct reconstructions' face does not change along
+  // the march, so neither does their table.  E_i loads it once; E_j and E_k
+  // (which keep three windows) load it per position, as holding it across
+  // the march leaves them too few registers at two waves per SIMD.
+  constexpr bool HOIST = UT && !VW;
+  R Tdu[HOIST ? GK_RECW_STRIDE : 1];
+  if constexpr (HOIST) {
+    const R *t = (sizeof(R) == 8 ? (const R *)a.geo.recw[DA] : (const R *)a.geo.recwf[DA]) +
+                 (long)idx[DA] * GK_RECW_STRIDE;
+#pragma unroll
+    for (int n = 0; n < GK_RECW_STRIDE; n++) Tdu[n] = t[n];
+  } else {
+    tw.init(a.geo, idx[DA], idx, f0);
+  }
   // staged form: LDS rows of the direct fields (E_i: v_a, v_b, then F1
   // current and half-step; E_j: F2 current and half-step)
   constexpr int NS = STAGED ? (VW ? 2 : 4) : 1;
@@ -1268,7 +1314,17 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       wv1.h[7] = V1[o];
     }
     march_load<1, R, R>(wb, Pw, Qw, o, cc, 7);
-    const R *Td = tw.at(f, f1);
+    const R *Td;
+    if constexpr (HOIST)
+      Td = Tdu;
+    else
+      Td = tw.at(f, f1);
+    R Tu[UT ? GK_RECW_STRIDE : 1];
+    const R *Tw = Tm;
+    if constexpr (UT) {
+      utab<MA, R>(a.geo, f, Tu);
+      Tw = Tu;
+    }
     R va, vb, b1l, b1r, b2l, b2r, l, rr;
     if constexpr (STAGED) {
       // stage the row: every global load first, then the LDS writes
@@ -1312,11 +1368,11 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
         pdm_lr(xh, xn, Td, L, Rr);
       };
       if (VW) {
-        lr_self(wv0, Tm, l, rr);
+        lr_self(wv0, Tw, l, rr);
         va = R(0.5) * (l + rr);
-        lr_self(wv1, Tm, l, rr);
+        lr_self(wv1, Tw, l, rr);
         vb = R(0.5) * (l + rr);
-        march_lr(wb, 0, Tm, b1l, b1r);
+        march_lr(wb, 0, Tw, b1l, b1r);
         lds_lr(0, 1, b2l, b2r);
       } else {
         lds_lr(0, 0, l, rr);
@@ -1324,15 +1380,15 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
         lds_lr(1, 1, l, rr);
         vb = R(0.5) * (l + rr);
         lds_lr(2, 3, b1l, b1r);
-        march_lr(wb, 0, Tm, b2l, b2r);
+        march_lr(wb, 0, Tw, b2l, b2r);
       }
       __syncthreads();  // the next position is staged over this one
     } else if (VW) {
-      lr_self(wv0, Tm, l, rr);
+      lr_self(wv0, Tw, l, rr);
       va = R(0.5) * (l + rr);
-      lr_self(wv1, Tm, l, rr);
+      lr_self(wv1, Tw, l, rr);
       vb = R(0.5) * (l + rr);
-      march_lr(wb, 0, Tm, b1l, b1r);
+      march_lr(wb, 0, Tw, b1l, b1r);
       recon(B2, B2p, c, g.st[R2], cc, Td, b2l, b2r);
     } else {
       const R *T2 = Td;
@@ -1341,7 +1397,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       recon(V1, (const R *)nullptr, c, g.st[A2], cc, T2, l, rr);
       vb = R(0.5) * (l + rr);
       recon(B1, B1p, c, g.st[A2], cc, T2, b1l, b1r);
-      march_lr(wb, 0, Tm, b2l, b2r);
+      march_lr(wb, 0, Tw, b2l, b2r);
     }
     const R w0 = rh(c - so), w1 = rh(c);
     // reference order: rho(c-sP-sQ) + rho(c-sP) + rho(c-sQ) + rho(c)
@@ -1365,23 +1421,26 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   }
 }
 
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
+    __attribute__((amdgpu_waves_per_eu(EmfWaves<D, R, UT>::lo, EmfWaves<D, R, UT>::hi))) void k_emf_march(EmfArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
-  if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_emf_march_body<D, R, false, B0V>(a, bx, by, bz);
+  // E_k reads its direct reconstruction across eight i rows: rows fastest,
+  // so that the blocks an XCD runs together share those rows in its L2 (the
+  // others run a row's march chunks together, which share window halos)
+  if (!xcd_block<D != 2>(gx, gy, gz, bx, by, bz)) return;
+  k_emf_march_body<D, R, false, B0V, UT>(a, bx, by, bz);
 }
 
 // staged form over whole 64-edge tiles along k (E_i, E_j): one wave per block
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(64) void k_emf_stage(EmfArgs a, int gx, int gy, int gz) {
   int bx, by, bz;
   if (!xcd_block<true>(gx, gy, gz, bx, by, bz)) return;
-  k_emf_march_body<D, R, true, B0V>(a, bx, by, bz);
+  k_emf_march_body<D, R, true, B0V, UT>(a, bx, by, bz);
 }
 
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restrict__ args,
                                                        const WorkItem *__restrict__ work,
                                                        int nwork) {
@@ -1391,13 +1450,13 @@ __global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restric
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R, true, B0V>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, true, B0V, UT>(args[arg], bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
-template <int D, typename R, bool B0V>
+template <int D, typename R, bool B0V, bool UT>
 __global__ __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(1, EmfWaves<D, R>::hi))) void k_emf_march_batch(const EmfArgs *__restrict__ args,
+    __attribute__((amdgpu_waves_per_eu(EmfWaves<D, R, UT>::lo, EmfWaves<D, R, UT>::hi))) void k_emf_march_batch(const EmfArgs *__restrict__ args,
                                                     const WorkItem *__restrict__ work, int nwork) {
   int w;
   if (!xcd_item(nwork, w)) return;
@@ -1406,7 +1465,7 @@ __global__ __launch_bounds__(256)
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R, false, B0V>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, false, B0V, UT>(args[arg], bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -2616,6 +2675,7 @@ struct gk_solver {
   int device;          // HIP device the solver was created on
   bool own_stream;     // the stream came from the pool (else the caller's)
   double *b0a;         // per-point background field (b0_mode 2), else null
+  bool ut;             // one reconstruction-table class per transverse axis
   int spherical;
   int bc[2][6];
   int ni, nj, nk;
@@ -2940,6 +3000,9 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
       s->geo.ncls[a][q] = d->ncls[a][q];
     }
   }
+  s->ut = true;
+  for (int a = 0; a < 3; a++)
+    for (int q = 0; q < 2; q++) s->ut = s->ut && d->ncls[a][q] == 1;
   int rc = GK_OK;
   auto fail = [&](int code) {
     gk_destroy(s);
@@ -3813,7 +3876,7 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
   }
 }
 
-template <typename R, bool B0V>
+template <typename R, bool B0V, bool UT>
 static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 blk(64, 4);
   const int gx = L.grid.x, gy = L.grid.y, gz = L.grid.z;
@@ -3823,16 +3886,16 @@ static void launch_one(const Launch &L, hipStream_t st) {
   const dim3 g1((total + NXCD - 1) / NXCD * NXCD);
   switch (L.kid) {
     case SK_BEGIN: k_begin<<<1, NSLOT, 0, st>>>(L.u.sc); break;
-    case SK_SWEEP_I: k_march<0, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_SWEEP_J: k_march<1, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_I: k_march<0, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_SWEEP_J: k_march<1, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_K: k_sweepk<R, B0V><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_KREM: k_kcol<R, B0V><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
-    case SK_EMF_I: k_emf_march<0, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_J: k_emf_march<1, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_K: k_emf_march<2, R, B0V><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_IS: k_emf_stage<0, R, B0V><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
-    case SK_EMF_JS: k_emf_stage<1, R, B0V><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_I: k_emf_march<0, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_J: k_emf_march<1, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_K: k_emf_march<2, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_IS: k_emf_stage<0, R, B0V, UT><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
+    case SK_EMF_JS: k_emf_stage<1, R, B0V, UT><<<g1, dim3(64), 0, st>>>(L.u.em, gx, gy, gz); break;
     case SK_CT: k_ct<R><<<g1, blk, 0, st>>>(L.u.ct, gx, gy, gz); break;
     case SK_CELL: k_cell<R, B0V><<<g1, blk, 0, st>>>(L.u.ce, gx, gy, gz); break;
     case SK_GHOST: k_ghost<<<g1, blk, 0, st>>>(L.u.gh, gx, gy, gz); break;
@@ -3847,22 +3910,22 @@ static void launch_one(const Launch &L, hipStream_t st) {
   }
 }
 
-template <typename R, bool B0V>
+template <typename R, bool B0V, bool UT>
 static void launch_batch(int kid, const void *args, const WorkItem *work, int nwork, int nblocks,
                          hipStream_t st) {
   const dim3 blk(64, 4);
   switch (kid) {
     case SK_BEGIN: k_begin_batch<<<nwork, NSLOT, 0, st>>>((Scal *const *)args, nwork); break;
-    case SK_SWEEP_I: k_march_batch<0, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_SWEEP_J: k_march_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_I: k_march_batch<0, R, B0V, UT><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_SWEEP_J: k_march_batch<1, R, B0V, UT><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_K: k_sweepk_batch<R, B0V><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_KREM: k_kcol_batch<R, B0V><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
-    case SK_EMF_I: k_emf_march_batch<0, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_J: k_emf_march_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_K: k_emf_march_batch<2, R, B0V><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_IS: k_emf_stage_batch<0, R, B0V><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
-    case SK_EMF_JS: k_emf_stage_batch<1, R, B0V><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_I: k_emf_march_batch<0, R, B0V, UT><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_J: k_emf_march_batch<1, R, B0V, UT><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_K: k_emf_march_batch<2, R, B0V, UT><<<nblocks, blk, 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_IS: k_emf_stage_batch<0, R, B0V, UT><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
+    case SK_EMF_JS: k_emf_stage_batch<1, R, B0V, UT><<<nblocks, dim3(64), 0, st>>>((const EmfArgs *)args, work, nwork); break;
     case SK_CT: k_ct_batch<R><<<nblocks, blk, 0, st>>>((const CtArgs *)args, work, nwork); break;
     case SK_CELL: k_cell_batch<R, B0V><<<nblocks, blk, 0, st>>>((const CellArgs *)args, work, nwork); break;
     case SK_GHOST: k_ghost_batch<<<nblocks, blk, 0, st>>>((const GhostArgs *)args, work, nwork); break;
@@ -3877,16 +3940,48 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
   }
 }
 
+// Launch groups: a solver's precision, whether it reads B0 per point, and
+// whether its reconstruction tables have one class per transverse axis
+// select the kernel instantiations it runs.
+constexpr int NGROUP = 8;
+
+static int launch_group(const gk_solver *s) {
+  return (s->precision == 1 ? 1 : 0) + (s->b0a ? 2 : 0) + (s->ut ? 4 : 0);
+}
+
+static void launch_one_group(int grp, const Launch &L, hipStream_t st) {
+  switch (grp) {
+    case 0: launch_one<double, false, false>(L, st); break;
+    case 1: launch_one<float, false, false>(L, st); break;
+    case 2: launch_one<double, true, false>(L, st); break;
+    case 3: launch_one<float, true, false>(L, st); break;
+    case 4: launch_one<double, false, true>(L, st); break;
+    case 5: launch_one<float, false, true>(L, st); break;
+    case 6: launch_one<double, true, true>(L, st); break;
+    default: launch_one<float, true, true>(L, st); break;
+  }
+}
+
+static void launch_batch_group(int grp, int kid, const void *args, const WorkItem *work, int nwork,
+                               int nblocks, hipStream_t st) {
+  switch (grp) {
+    case 0: launch_batch<double, false, false>(kid, args, work, nwork, nblocks, st); break;
+    case 1: launch_batch<float, false, false>(kid, args, work, nwork, nblocks, st); break;
+    case 2: launch_batch<double, true, false>(kid, args, work, nwork, nblocks, st); break;
+    case 3: launch_batch<float, true, false>(kid, args, work, nwork, nblocks, st); break;
+    case 4: launch_batch<double, false, true>(kid, args, work, nwork, nblocks, st); break;
+    case 5: launch_batch<float, false, true>(kid, args, work, nwork, nblocks, st); break;
+    case 6: launch_batch<double, true, true>(kid, args, work, nwork, nblocks, st); break;
+    default: launch_batch<float, true, true>(kid, args, work, nwork, nblocks, st); break;
+  }
+}
+
 static void one_step(gk_solver *s) {
   std::vector<Launch> &plan = s->plan;
   plan_step(s, rot_of(s), plan);
   for (const Launch &L : plan) {
     Timer t(s, sk_name[L.kid]);
-    const bool v = s->b0a != nullptr;
-    if (s->precision == 1)
-      v ? launch_one<float, true>(L, s->stream) : launch_one<float, false>(L, s->stream);
-    else
-      v ? launch_one<double, true>(L, s->stream) : launch_one<double, false>(L, s->stream);
+    launch_one_group(launch_group(s), L, s->stream);
   }
   const Rot r = rot_next(rot_of(s));
   s->cur = r.cur;
@@ -4205,12 +4300,6 @@ extern "C" int gk_get_tuning(const gk_solver *s, int param) {
 // work list covering every grid in the batch (grids of one precision share a
 // launch); the launches of a step are captured in HIP graphs.  Each grid keeps
 // its own dt, time and history, and runs exactly the arithmetic of gk_step.
-constexpr int NGROUP = 4;  // launch groups of a batch (precision x per-point B0)
-
-static int launch_group(const gk_solver *s) {
-  return (s->precision == 1 ? 1 : 0) + (s->b0a ? 2 : 0);
-}
-
 struct gk_batch {
   std::vector<gk_solver *> mem;
   hipStream_t stream = nullptr;
@@ -4349,19 +4438,14 @@ static void batch_launch_step(gk_batch *b, int p, bool timed) {
       }
       const void *args = b->dargs + b->args_off[gp][p][k];
       const WorkItem *work = b->dwork + b->work_off[gp][k];
-      switch (gp) {
-        case 0: launch_batch<double, false>(k, args, work, nw, nblocks, b->stream); break;
-        case 1: launch_batch<float, false>(k, args, work, nw, nblocks, b->stream); break;
-        case 2: launch_batch<double, true>(k, args, work, nw, nblocks, b->stream); break;
-        default: launch_batch<float, true>(k, args, work, nw, nblocks, b->stream); break;
-      }
+      launch_batch_group(gp, k, args, work, nw, nblocks, b->stream);
       if (timed) {
         (void)hipEventRecord(e1, b->stream);
         (void)hipEventSynchronize(e1);
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         std::string nm = std::string(sk_name[k]) + ((gp & 1) ? "/f32" : "/f64") +
-                         ((gp & 2) ? "/b0" : "");
+                         ((gp & 2) ? "/b0" : "") + ((gp & 4) ? "/ut" : "");
         int slot = -1;
         for (size_t q = 0; q < b->names.size(); q++)
           if (b->names[q] == nm) slot = (int)q;
