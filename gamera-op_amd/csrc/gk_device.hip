@@ -1288,7 +1288,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
   // the march, so neither does their table.  E_i loads it once; E_j and E_k
   // (which keep three windows) load it per position, as holding it across
   // the march leaves them too few registers at two waves per SIMD.
-  constexpr bool HOIST = UT && !VW;
+  constexpr bool HOIST = UT && (!VW || STAGED);
   R Tdu[HOIST ? GK_RECW_STRIDE : 1];
   if constexpr (HOIST) {
     const R *t = (sizeof(R) == 8 ? (const R *)a.geo.recw[DA] : (const R *)a.geo.recwf[DA]) +
@@ -2706,6 +2706,7 @@ struct gk_solver {
   int timing;
   int chunk;           // faces per thread in the marching sweeps
   int emf_chunk;       // edges per thread in the marching EMF kernels
+  int emf_stage;       // bit D: E_D (D = 0, 1) over whole 64-edge k tiles staged in LDS
   double rho_floor, p_floor;  // positivity floors
   double grav[3];      // uniform gravity
   struct gk_batch *batch;  // batch that owns the stepping, if any
@@ -3119,6 +3120,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
   s->chunk = 32;
   s->emf_chunk = 16;
+  s->emf_stage = s->precision == 1 ? 2 : 3;  // measured fastest at 256^3
   s->cur = 0;
   s->b_m1 = 0;
   s->b_0 = 1;
@@ -3687,10 +3689,10 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     const int MA = D == 1 ? 0 : 1, RA = D == 1 ? 1 : 0;
     a.nch = march_chunks(a.hi[MA] - a.lo[MA] + 1, s->emf_chunk);
     const int nf = a.nch;
-    // FP64 E_i: whole 64-edge tiles along k in the staged form, the rest (all
-    // of it when there is no whole tile) in the marching form.  Staging
-    // measured slower for E_j (one direct field) and for the FP32 form.
-    const int ntile = (D == 0 && !f32) ? (a.hi[2] - a.lo[2] + 1) / 64 : 0;
+    // staged E_i / E_j (bit D of emf_stage; by default both in FP64, E_j in
+    // FP32-bulk): whole 64-edge tiles along k in the staged form, the rest
+    // (all of it when there is no whole tile) in the marching form
+    const int ntile = (D < 2 && ((s->emf_stage >> D) & 1)) ? (a.hi[2] - a.lo[2] + 1) / 64 : 0;
     if (ntile > 0) {
       EmfArgs t = a;
       t.hi[2] = a.lo[2] + 64 * ntile - 1;
@@ -4275,8 +4277,13 @@ extern "C" int gk_copy_calibrate(size_t bytes, int reps, double *ms) {
 
 extern "C" int gk_set_tuning(gk_solver *s, int param, int value) {
   if (!s) return GK_ERR_ARG;
-  if (value < 1 || value > 4096) return GK_ERR_ARG;
   if (s->batch) return GK_ERR_STATE;  // a batch's launch tables hold its shapes
+  if (param == GK_TUNE_EMF_STAGE) {
+    if (value < 0 || value > 3) return GK_ERR_ARG;
+    s->emf_stage = value;
+    return GK_OK;
+  }
+  if (value < 1 || value > 4096) return GK_ERR_ARG;
   if (param == GK_TUNE_CHUNK) {
     s->chunk = value;
     return GK_OK;
@@ -4292,6 +4299,7 @@ extern "C" int gk_get_tuning(const gk_solver *s, int param) {
   if (!s) return GK_ERR_ARG;
   if (param == GK_TUNE_CHUNK) return s->chunk;
   if (param == GK_TUNE_EMF_CHUNK) return s->emf_chunk;
+  if (param == GK_TUNE_EMF_STAGE) return s->emf_stage;
   return GK_ERR_ARG;
 }
 
@@ -4687,7 +4695,7 @@ struct TuneKey {
   }
 };
 struct TunePick {
-  int chunk, emf_chunk;
+  int chunk, emf_chunk, emf_stage;
 };
 static std::vector<std::pair<TuneKey, TunePick>> g_tune_cache;
 
@@ -4699,6 +4707,7 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
     if (e.first == key) {
       s->chunk = e.second.chunk;
       s->emf_chunk = e.second.emf_chunk;
+      s->emf_stage = e.second.emf_stage;
       if (best) *best = e.second.chunk;
       return GK_OK;
     }
@@ -4768,14 +4777,15 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
     }
     return snap(true);
   };
-  // march length of the sweeps, then of the edge-field kernels.  Candidates
-  // go from short to long marches (more to fewer threads); a longer one must
-  // be 3% faster to be taken, so near-ties keep the more parallel shape, which
-  // also suits a grid stepped inside a batch
+  // march length of the sweeps, then of the edge-field kernels, then which
+  // of E_i and E_j run in the LDS-staged form.  Candidates go from short to
+  // long marches (more to fewer threads); a longer one must be 3% faster to
+  // be taken, so near-ties keep the more parallel shape, which also suits a
+  // grid stepped inside a batch
   const float kPrefer = 0.97f;
   const int cand[] = {8, 16, 32, 64, 128, 256};
   const int cand_e[] = {2, 4, 8, 16, 32};
-  int pick = s->chunk, pick_e = s->emf_chunk;
+  int pick = s->chunk, pick_e = s->emf_chunk, pick_s = s->emf_stage;
   float best_ms = 1e30f;
   for (int c : cand) {
     if (rc != GK_OK) break;
@@ -4801,6 +4811,24 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
       pick_e = c;
     }
   }
+  s->emf_chunk = pick_e;
+  // the staged forms cover whole 64-edge tiles along k: without one, every
+  // setting launches the same kernels
+  if (s->nk + 1 >= 64) {
+    best_ms = 1e30f;
+    const int dflt = s->emf_stage;
+    for (int c : {dflt, 0, 1, 2, 3}) {
+      if (rc != GK_OK) break;
+      if (c == dflt && best_ms < 1e30f) continue;
+      s->emf_stage = c;
+      float ms = 0.0f;
+      if ((rc = trial(ms)) != GK_OK) break;
+      if (ms < best_ms * kPrefer) {
+        best_ms = ms;
+        pick_s = c;
+      }
+    }
+  }
   s->timing = keep_timing;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -4809,7 +4837,8 @@ extern "C" int gk_autotune(gk_solver *s, int trial_steps, int *best) {
   if (rc != GK_OK) return rc;
   s->chunk = pick;
   s->emf_chunk = pick_e;
-  g_tune_cache.push_back({key, TunePick{pick, pick_e}});
+  s->emf_stage = pick_s;
+  g_tune_cache.push_back({key, TunePick{pick, pick_e, pick_s}});
   if (best) *best = pick;
   return GK_OK;
 }

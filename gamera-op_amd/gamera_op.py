@@ -188,6 +188,7 @@ def copy_calibrate(nbytes, reps=20):
 
 TUNE_CHUNK = 0
 TUNE_EMF_CHUNK = 1
+TUNE_EMF_STAGE = 2
 
 
 class GkError(RuntimeError):

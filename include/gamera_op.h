@@ -211,15 +211,20 @@ size_t gk_device_bytes(const gk_solver *s);
    that on grids large enough to fill the GPU with it); GK_TUNE_EMF_CHUNK:
    about how many edges each thread walks in the edge-field kernels
    (1..4096, default 16).  A column is split into near-equal chunks of
-   about that length (within 1.5x of it).  GK_ERR_STATE while the solver is
-   in a batch (the batch's launch tables hold its shapes). */
-enum { GK_TUNE_CHUNK = 0, GK_TUNE_EMF_CHUNK = 1 };
+   about that length (within 1.5x of it).  GK_TUNE_EMF_STAGE: bit D (D = 0
+   for E_i, 1 for E_j) runs that edge field over whole 64-edge tiles along
+   k with its k-direction reconstructions staged in LDS (0..3; default 3 in
+   FP64, 2 in FP32-bulk).  GK_ERR_STATE while the solver is in a batch (the
+   batch's launch tables hold its shapes). */
+enum { GK_TUNE_CHUNK = 0, GK_TUNE_EMF_CHUNK = 1, GK_TUNE_EMF_STAGE = 2 };
 int gk_set_tuning(gk_solver *s, int param, int value);
 int gk_get_tuning(const gk_solver *s, int param);
 
 /* Launch-shape autotuner (no counterpart in the reference): times
-   trial_steps steps for each candidate march length on this grid, keeps the
-   fastest (written to *best) and restores the state it started from, so
+   trial_steps steps for each candidate march length of the sweeps and of the
+   edge-field kernels and, on grids with whole 64-edge tiles along k, each
+   GK_TUNE_EMF_STAGE setting; keeps the fastest of each (the sweeps' march
+   length is written to *best) and restores the state it started from, so
    results do not depend on tuning.  Picks are cached per grid shape and
    precision for the life of the process.  trial_steps = 0 only applies a
    cached pick (*best = 0 when there is none) and times nothing. */

@@ -67,3 +67,37 @@ def test_device_matches_oracle(kind, seed, ni, nj, nk, steps):
         assert abs(sc["dt"] - O.scalar("dt")) <= 1e-10 * O.scalar("dt")
     finally:
         P.close()
+
+
+# launch shapes change speed, never results: every staged/marching split of
+# the edge fields and a range of march lengths, on grids with one whole
+# 64-edge k tile plus leftover columns (Cartesian, shell, per-point B0)
+SHAPE_CASES = [
+    (go.SINUSOID, 6, 8, 12, 64, 6),
+    (go.DIPOLE_SHELL, 7, 12, 8, 70, 6),
+    (go.VARYING_B0_BOX, 43, 12, 8, 66, 6),
+]
+SHAPES = [(32, 16, 0), (32, 16, 2), (32, 16, 3), (8, 4, 1), (64, 2, 3)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed,ni,nj,nk,steps", SHAPE_CASES)
+def test_launch_shapes_match_oracle(kind, seed, ni, nj, nk, steps):
+    print(f"shapes kind={kind} seed={seed} {ni}x{nj}x{nk} steps={steps}", flush=True)
+    P = go.Problem(kind, seed, ni, nj, nk, full_b0=True)
+    try:
+        O = run_oracle(P, steps)
+        for chunk, emf_chunk, stage in SHAPES:
+            print("shape", chunk, emf_chunk, stage, flush=True)
+            P.reseed(seed)
+            P.device_open(0)
+            P.set_tuning(go.TUNE_CHUNK, chunk)
+            P.set_tuning(go.TUNE_EMF_CHUNK, emf_chunk)
+            P.set_tuning(go.TUNE_EMF_STAGE, stage)
+            assert P.get_tuning(go.TUNE_EMF_STAGE) == stage
+            P.device_step(steps)
+            P.device_fetch()
+            assert exact_mismatches(P, O) == 0, (chunk, emf_chunk, stage)
+            assert P.device_scalars()["time"] == O.scalar("time_sim")
+    finally:
+        P.close()
