@@ -521,6 +521,41 @@ struct WorkItem {
   int arg, bx, by, bz;
 };
 
+// A batched launch reads each grid's argument set from global memory, so the
+// compiler cannot tell that the pointers in it address global memory and
+// would access them with flat instructions (slower to issue, and counted
+// against the LDS counter too, so waits on scalar and LDS loads also wait for
+// them).  A batched kernel copies its set and marks every pointer in it as a
+// global-memory pointer (an integer round trip, which the optimiser does not
+// fold away), as it does by itself for a kernel argument's pointers.
+template <typename T>
+__device__ __forceinline__ void gptr(T *&p) {
+  p = (T *)(__attribute__((address_space(1))) T *)(unsigned long)p;
+}
+
+__device__ __forceinline__ void globalize(Geo &g) {
+#pragma unroll
+  for (int q = 0; q < GK_G_COUNT; q++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) gptr(g.g[q][a]);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) gptr(g.cc[a][b]);
+    gptr(g.recw[a]);
+    gptr(g.recwf[a]);
+    gptr(g.cls[a][0]);
+    gptr(g.cls[a][1]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T batch_args(const T *args, int arg) {
+  T a = args[arg];
+  globalize(a);
+  return a;
+}
+
 // Marching kernels split the march range [lo, hi] into nch near-equal chunks
 // (lengths differ by at most one), one per thread; chunk bz is [f0, f1).
 // Equal shares avoid a nearly empty last chunk (257 faces at 32 per thread
@@ -554,6 +589,19 @@ struct SweepArgs {
   int flo[3], fhi[3];    // range where the full flux is needed
   int nch;               // marching sweeps: near-equal chunks of the march range
 };
+
+__device__ __forceinline__ void globalize(SweepArgs &a) {
+  globalize(a.geo);
+  gptr(a.b0a);
+  gptr(a.X);
+  gptr(a.Xp);
+  gptr(a.M);
+  gptr(a.Mp);
+  gptr(a.B);
+  gptr(a.F);
+  gptr(a.V);
+  gptr(a.sc);
+}
 
 // Direct sweep for the velocity-only strips of the j sweep (k in the four
 // ghost layers on either side).  Each thread reads its eight stencil cells;
@@ -626,7 +674,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const SweepArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_sweep_body<D, R, B0V>(args[arg], bx, by, bz);
+  k_sweep_body<D, R, B0V>(batch_args(args, arg), bx, by, bz);
 }
 
 // Marching sweep along D = 0 (i) or 1 (j).  Lanes run along k (coalesced),
@@ -745,7 +793,7 @@ __global__ __launch_bounds__(64) void k_kcol_batch(const SweepArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_kcol_body<R, B0V>(args[arg], bx, by, bz);
+  k_kcol_body<R, B0V>(batch_args(args, arg), bx, by, bz);
 }
 
 // Points of a marching launch: the (row, k) plane of the launch box is walked
@@ -945,7 +993,7 @@ __global__ __launch_bounds__(256) void k_march_batch(const SweepArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_march_body<D, R, B0V, UT>(args[arg], bx, by, bz);
+  k_march_body<D, R, B0V, UT>(batch_args(args, arg), bx, by, bz);
 }
 
 // k sweep (D = 2) with the stencil staged in LDS.  A block is one wave
@@ -1077,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_sweepk_batch(const SweepArgs *__restric
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_sweepk_body<R, B0V>(args[arg], bx, by, bz);
+  k_sweepk_body<R, B0V>(batch_args(args, arg), bx, by, bz);
 }
 
 // Edge EMF from the edge velocities (va, vb), the two reconstructed face-B
@@ -1133,6 +1181,18 @@ struct EmfArgs {
   int lo[3], hi[3];
   int nch;        // near-equal chunks of the march range (one per thread)
 };
+
+__device__ __forceinline__ void globalize(EmfArgs &a) {
+  globalize(a.geo);
+  gptr(a.b0a);
+  gptr(a.X);
+  gptr(a.Xp);
+  gptr(a.B);
+  gptr(a.Bp);
+  gptr(a.V);
+  gptr(a.E);
+  gptr(a.sc);
+}
 
 // Marching EMF along the axis MA that two of the three reconstructions share
 // (E_i: j, with B_k windowed and the k-reconstructions of v and B_j read
@@ -1450,7 +1510,7 @@ __global__ __launch_bounds__(64) void k_emf_stage_batch(const EmfArgs *__restric
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R, true, B0V, UT>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, true, B0V, UT>(batch_args(args, arg), bx, by, bz);
 }
 
 // one launch for many grids: block w runs work-list entry w (argument set, block)
@@ -1465,7 +1525,7 @@ __global__ __launch_bounds__(256)
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_emf_march_body<D, R, false, B0V, UT>(args[arg], bx, by, bz);
+  k_emf_march_body<D, R, false, B0V, UT>(batch_args(args, arg), bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1478,6 +1538,15 @@ struct CtArgs {
   const void *E;     // 3 edge fields of R
   const Scal *sc;
 };
+
+__device__ __forceinline__ void globalize(CtArgs &a) {
+  globalize(a.geo);
+  gptr(a.B);
+  gptr(a.Bn);
+  gptr(a.Bn32);
+  gptr(a.E);
+  gptr(a.sc);
+}
 
 // Constrained-transport face update (reference src/solver_solve.c:262-306),
 // in FP64 for both forms (EMFs of type R are widened first), so the discrete
@@ -1552,7 +1621,7 @@ __global__ __launch_bounds__(256) void k_ct_batch(const CtArgs *__restrict__ arg
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_ct_body<R>(args[arg], bx, by, bz);
+  k_ct_body<R>(batch_args(args, arg), bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1577,6 +1646,24 @@ struct CellArgs {
                            // the fused dt / div B reduction (pole ring rows,
                            // reduced again after the ring average); empty if lo > hi
 };
+
+__device__ __forceinline__ void globalize(CellArgs &a) {
+  globalize(a.geo);
+  gptr(a.b0a);
+  gptr(a.X);
+  gptr(a.Xp);
+  gptr(a.Xn);
+  gptr(a.Xn32);
+  gptr(a.M);
+  gptr(a.Mp);
+  gptr(a.Mn);
+  gptr(a.Mn32);
+  gptr(a.Bn);
+  gptr(a.F[0]);
+  gptr(a.F[1]);
+  gptr(a.F[2]);
+  gptr(a.sc);
+}
 
 // Cell update (reference src/solver_solve.c:178-255, 314-484; positivity
 // src/solver_func.c:2117-2135; spherical sources
@@ -1814,7 +1901,7 @@ __global__ __launch_bounds__(256) void k_cell_batch(const CellArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_cell_body<R, B0V>(args[arg], bx, by, bz);
+  k_cell_body<R, B0V>(batch_args(args, arg), bx, by, bz);
 }
 
 // CFL dt and |div B| of the current state (prologue; src/solver_func.c:255-343)
@@ -1828,6 +1915,16 @@ struct DtArgs {
   int j0, j1;          // rows reduced (blockIdx.y covers them from j0)
   Scal *sc;
 };
+
+__device__ __forceinline__ void globalize(DtArgs &a) {
+  globalize(a.geo);
+  gptr(a.b0a);
+  gptr(a.X);
+  gptr(a.M);
+  gptr(a.B);
+  gptr(a.dxkj);
+  gptr(a.sc);
+}
 
 template <bool B0V>
 __device__ __forceinline__ void k_dt_body(const DtArgs &a, const int bx, const int by,
@@ -1885,7 +1982,7 @@ __global__ __launch_bounds__(256) void k_dt_batch(const DtArgs *__restrict__ arg
   int w;
   if (!xcd_item(nwork, w)) return;
   const int arg = __builtin_amdgcn_readfirstlane(work[w].arg);
-  k_dt_body<B0V>(args[arg], __builtin_amdgcn_readfirstlane(work[w].bx),
+  k_dt_body<B0V>(batch_args(args, arg), __builtin_amdgcn_readfirstlane(work[w].bx),
       __builtin_amdgcn_readfirstlane(work[w].by), __builtin_amdgcn_readfirstlane(work[w].bz));
 }
 
@@ -1908,6 +2005,14 @@ struct GhostArgs {
   int bc[2][6];
   const Scal *sc;  // run control of a step's launch (null outside a step)
 };
+
+__device__ __forceinline__ void globalize(GhostArgs &a) {
+  gptr(a.X);
+  gptr(a.B);
+  gptr(a.X32);
+  gptr(a.B32);
+  gptr(a.sc);
+}
 
 __device__ __forceinline__ bool mirror_kind(int kind) {
   return kind == GK_BC_SYMMETRIC || kind == GK_BC_POLE || kind == GK_BC_REFLECTIVE;
@@ -2121,7 +2226,7 @@ __global__ __launch_bounds__(256) void k_ghost_batch(const GhostArgs *__restrict
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_ghost_body(args[arg], bx, by, bz);
+  k_ghost_body(batch_args(args, arg), bx, by, bz);
 }
 
 // cell-centred B over cells 0..N-2 of each axis (get_derived_variables,
@@ -2134,6 +2239,14 @@ struct MagsArgs {
   float *M32;  // FP32 copy (FP32-bulk form), else null
   const Scal *sc;  // run control of a step's launch (null outside a step)
 };
+
+__device__ __forceinline__ void globalize(MagsArgs &a) {
+  globalize(a.geo);
+  gptr(a.B);
+  gptr(a.M);
+  gptr(a.M32);
+  gptr(a.sc);
+}
 
 template <bool SHELL = false>
 __device__ __forceinline__ void k_mags_body(const MagsArgs &a, const int bx, const int by,
@@ -2210,7 +2323,7 @@ __global__ __launch_bounds__(256) void k_mags_batch(const MagsArgs *__restrict__
   const int bx = __builtin_amdgcn_readfirstlane(work[w].bx);
   const int by = __builtin_amdgcn_readfirstlane(work[w].by);
   const int bz = __builtin_amdgcn_readfirstlane(work[w].bz);
-  k_mags_body<true>(args[arg], bx, by, bz);
+  k_mags_body<true>(batch_args(args, arg), bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -4359,6 +4472,54 @@ static void batch_free_tables(gk_batch *b) {
   b->nflags = 0;
 }
 
+// A grid's blocks of one stage in the logical order its own launch deals
+// them to the XCDs (the decode of xcd_block / xcd_patch in that stage's
+// kernel).  A batched launch deals consecutive work-list entries to the same
+// XCD, so walking each grid in this order keeps the blocks that share halos,
+// chunk prologues and faces in one L2, as in a launch of the grid alone.
+static void solo_order(int kid, dim3 gr, int idx, std::vector<WorkItem> &out) {
+  const int gx = (int)gr.x, gy = (int)gr.y, gz = (int)gr.z;
+  int mode = 0;  // 0: x fastest (xcd_block<false>), 1: z fastest (xcd_block<true>), 2: patches
+  switch (kid) {
+    case SK_SWEEP_I: case SK_SWEEP_J: case SK_EMF_I: case SK_EMF_J: case SK_EMF_IS:
+    case SK_EMF_JS: case SK_GHOST:
+      mode = 1;
+      break;
+    case SK_CT: case SK_CELL:
+      mode = gy == 1 ? 2 : 0;
+      break;
+    default:  // SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_K, SK_MAGS, the ring stages
+      mode = 0;
+  }
+  if (mode == 2) {
+    const int npz = (gz + PATCH - 1) / PATCH;
+    const long total = (long)(gx + PATCH - 1) / PATCH * PATCH * npz * PATCH;
+    for (long L = 0; L < total; L++) {
+      const long pp = L / (PATCH * PATCH), w = L % (PATCH * PATCH);
+      const int bz = (int)(pp % npz * PATCH + w % PATCH);
+      const int bx = (int)(pp / npz * PATCH + w / PATCH);
+      if (bx < gx && bz < gz) out.push_back(WorkItem{idx, bx, 0, bz});
+    }
+    return;
+  }
+  const long total = (long)gx * gy * gz;
+  for (long L = 0; L < total; L++) {
+    int bx, by, bz;
+    if (mode == 1) {
+      bz = (int)(L % gz);
+      const long t = L / gz;
+      bx = (int)(t % gx);
+      by = (int)(t / gx);
+    } else {
+      bx = (int)(L % gx);
+      const long t = L / gx;
+      by = (int)(t % gy);
+      bz = (int)(t / gy);
+    }
+    out.push_back(WorkItem{idx, bx, by, bz});
+  }
+}
+
 static int batch_build(gk_batch *b) {
   HIPCHK(hipStreamSynchronize(b->stream));  // nothing in flight uses the old tables
   batch_free_tables(b);
@@ -4376,11 +4537,7 @@ static int batch_build(gk_batch *b) {
         const int idx = (int)(A.size() / sz);
         const char *src = (const char *)stage_arg_ptr(L.kid, L.u);
         A.insert(A.end(), src, src + sz);
-        if (p == 0 && L.kid != SK_BEGIN)
-          for (unsigned z = 0; z < L.grid.z; z++)
-            for (unsigned y = 0; y < L.grid.y; y++)
-              for (unsigned x = 0; x < L.grid.x; x++)
-                hwork[gp][L.kid].push_back(WorkItem{idx, (int)x, (int)y, (int)z});
+        if (p == 0 && L.kid != SK_BEGIN) solo_order(L.kid, L.grid, idx, hwork[gp][L.kid]);
       }
       r = rot_next(r);
     }
