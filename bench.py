@@ -342,6 +342,9 @@ def run_serving(args, preparer):
     srv = serve.summarize(res, wall)
     srv["tuning_s_before_stream"] = round(tuning, 3)
     srv["loop_seconds"] = {k: round(v, 3) for k, v in serve.serve.last_stats.items()}
+    # device time of the batch's quanta over the stream's wall time (the
+    # members' set-up and read-back on the side stream are not counted)
+    srv["gpu_busy_share"] = serve.serve.last_stats.get("gpu_busy_s", 0.0) / wall
     log(f"bench: served {srv}")
     log("bench: plain loop on the same requests")
     res_p, wall_p = serve.serve_plain(reqs)

@@ -4448,7 +4448,21 @@ struct gk_batch {
   unsigned int *dflags = nullptr;
   unsigned int *hflags = nullptr;
   int nflags = 0;
+  // device time of runs (gk_batch_run_timing): events around the last run
+  hipEvent_t run_ev[2] = {};
+  bool run_on = false, run_pending = false;
+  double run_ms = 0.0;
 };
+
+// fold the last run's device time into the total once its end event is done
+static void run_fold(gk_batch *b) {
+  if (!b->run_pending) return;
+  float ms = 0.0f;
+  if (hipEventSynchronize(b->run_ev[1]) == hipSuccess &&
+      hipEventElapsedTime(&ms, b->run_ev[0], b->run_ev[1]) == hipSuccess)
+    b->run_ms += ms;
+  b->run_pending = false;
+}
 
 static void batch_free_tables(gk_batch *b) {
   for (int p = 0; p < 6; p++)
@@ -4673,6 +4687,8 @@ extern "C" int gk_batch_destroy(gk_batch *b) {
   (void)hipStreamSynchronize(b->stream);
   for (gk_solver *s : b->mem) s->batch = nullptr;
   batch_free_tables(b);
+  for (hipEvent_t e : b->run_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(b->stream);
   // solvers created on the side stream may outlive the batch: the stream
   // stays alive in the pool
@@ -4773,6 +4789,10 @@ extern "C" int gk_batch_step_async(gk_batch *b, int n) {
       b->acc_n[q] = 0;
     }
   }
+  if (b->run_on) {
+    run_fold(b);
+    HIPCHK(hipEventRecord(b->run_ev[0], b->stream));
+  }
   int left = n;
   while (left > 0) {
     if (b->timing || !b->use_graphs) {
@@ -4804,6 +4824,10 @@ extern "C" int gk_batch_step_async(gk_batch *b, int n) {
     HIPCHK(hipMemcpyAsync(b->hflags, b->dflags, b->nflags * sizeof(unsigned int),
                           hipMemcpyDeviceToHost, b->stream));
   }
+  if (b->run_on) {
+    HIPCHK(hipEventRecord(b->run_ev[1], b->stream));
+    b->run_pending = true;
+  }
   // every grid's history rotation advances by the steps it takes: n, or
   // what is left of its budget (the device stops it there; its later
   // launches do nothing).  A member that stopped short is out of phase with
@@ -4829,8 +4853,29 @@ extern "C" int gk_batch_step_async(gk_batch *b, int n) {
 extern "C" int gk_batch_sync(gk_batch *b) {
   if (!b) return GK_ERR_ARG;
   HIPCHK(hipStreamSynchronize(b->stream));
+  run_fold(b);
   for (int q = 0; q < b->nflags; q++)
     if (b->hflags[q]) return GK_ERR_NAN;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_run_timing(gk_batch *b, int on) {
+  if (!b) return GK_ERR_ARG;
+  if (on && !b->run_ev[0]) {
+    for (hipEvent_t &e : b->run_ev) HIPCHK(hipEventCreate(&e));
+  }
+  if (on) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->run_pending = false;
+    b->run_ms = 0.0;
+  }
+  b->run_on = on != 0;
+  return GK_OK;
+}
+
+extern "C" int gk_batch_run_ms(gk_batch *b, double *ms) {
+  if (!b || !ms) return GK_ERR_ARG;
+  *ms = b->run_ms;
   return GK_OK;
 }
 

@@ -109,6 +109,8 @@ def _load():
                        ("gk_batch_sync", [vp]),
                        ("gk_batch_set_graphs", [vp, ctypes.c_int]),
                        ("gk_batch_side_stream", [vp, ctypes.POINTER(vp)]),
+                       ("gk_batch_run_timing", [vp, ctypes.c_int]),
+                       ("gk_batch_run_ms", [vp, ctypes.POINTER(ctypes.c_double)]),
                        ("gk_batch_enable_timing", [vp, ctypes.c_int]),
                        ("gk_batch_kernel_times", [vp, ctypes.POINTER(ctypes.c_char_p),
                                                   ctypes.POINTER(ctypes.c_double),
@@ -488,6 +490,16 @@ class Batch:
         st = ctypes.c_void_p()
         _check(self.op.gk_batch_side_stream(self.handle, ctypes.byref(st)), "gk_batch_side_stream")
         return st
+
+    def run_timing(self, on=True):
+        """Time each run on the device (gk_batch_run_timing); resets the total."""
+        _check(self.op.gk_batch_run_timing(self.handle, 1 if on else 0), "gk_batch_run_timing")
+
+    def run_ms(self):
+        """Device milliseconds of the completed runs since run timing was turned on."""
+        ms = ctypes.c_double()
+        _check(self.op.gk_batch_run_ms(self.handle, ctypes.byref(ms)), "gk_batch_run_ms")
+        return ms.value
 
     def kernel_times(self):
         n = ctypes.c_int(64)

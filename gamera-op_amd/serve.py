@@ -261,6 +261,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
     setup = _Setup() if preparer is None else None
     ahead = lookahead or (2 * preparer.workers if preparer is not None else 0)
     B = go.Batch()
+    B.run_timing(True)  # device time of the quanta: the GPU-busy share
     side = B.side_stream()
     queue = collections.deque()      # arrived, not yet being prepared
     preparing = collections.deque()  # (request, async result) in arrival order
@@ -387,6 +388,7 @@ def serve(requests, budget_bytes=64 << 30, max_slots=64, quantum=25, tune=True, 
             elif nxt < len(reqs) and not queue:
                 time.sleep(max(0.0, min(reqs[nxt].arrival - now(), 0.05)))
         wall = now()
+        stats["gpu_busy_s"] = B.run_ms() / 1e3
     finally:
         B.close()
         for a in active + pending:

@@ -265,6 +265,16 @@ int gk_batch_set_graphs(gk_batch *b, int on); /* default on */
    be set up, read back and destroyed while the batch steps, without waiting
    for its launched steps.  It stays valid after gk_batch_destroy. */
 int gk_batch_side_stream(gk_batch *b, void **stream);
+
+/* Device time of the batch's runs (no counterpart in the reference; for the
+   serving loop's GPU-busy share).  With run timing on, each gk_batch_step /
+   gk_batch_step_async call is timed on the batch's stream from its first
+   launch to the read-back of its non-finite flags; gk_batch_run_ms gives the
+   milliseconds of the runs completed (synchronised) since it was turned on
+   (turning it on resets the total).  Work on the side stream is not
+   counted. */
+int gk_batch_run_timing(gk_batch *b, int on);
+int gk_batch_run_ms(gk_batch *b, double *ms);
 int gk_batch_enable_timing(gk_batch *b, int on);
 int gk_batch_kernel_times(gk_batch *b, const char **names, double *ms, int *n);
 
