@@ -323,6 +323,43 @@ def run_ensemble(args):
             "sizes": "drawn from " + str(ENSEMBLE_SIZES)}
 
 
+def run_ensemble_sizes(args):
+    """Same-size batches of each ENSEMBLE_SIZES shape (about 8.4M cells each,
+    half sinusoid, half sinusoid + sphere, FP32-bulk, autotuned) and one
+    256^3 grid stepped as a one-member batch: the batched rate each size
+    reaches on its own, which bounds the mixed ensemble's rate."""
+    sys.path.insert(0, PKG)
+    import gamera_op as go
+    from ensemble import ENSEMBLE_SIZES
+    out = []
+    for shape in ENSEMBLE_SIZES + [(256, 256, 256)]:
+        cells1 = shape[0] * shape[1] * shape[2]
+        n = 1 if shape == (256, 256, 256) else max(1, (8 << 20) // cells1)
+        solvers = []
+        for q in range(n):
+            P = go.Problem(go.SINUSOID_SPHERE if q % 2 else go.SINUSOID, 100 + q, *shape)
+            S = go.Solver(P, 1)
+            P.close()
+            S.autotune(2)
+            solvers.append(S)
+        B = go.Batch()
+        for S in solvers:
+            B.add(S)
+        B.step(6)
+        steps = 60
+        t0 = time.perf_counter()
+        B.step(steps)
+        el = time.perf_counter() - t0
+        B.close()
+        for S in solvers:
+            S.close()
+        row = {"shape": list(shape), "grids": n, "cells": n * cells1,
+               "ms_per_step": el * 1e3 / steps, "cell_updates_per_s": n * cells1 * steps / el}
+        log(f"bench: same-size batch {row}")
+        out.append(row)
+    return out
+
+
 def run_serving(args, preparer):
     """Served stream (scaled): seeded Poisson arrivals, 70% Cartesian
     (sinusoid + sphere, 32^3..128^3) / 30% spherical shell (dipole + blob),
@@ -502,7 +539,7 @@ def main():
     ap.add_argument("--ring-dims", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--ring-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["serve", "ensemble", "solve"], default=None,
+    ap.add_argument("--only", choices=["serve", "ensemble", "solve", "sizes"], default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -519,9 +556,11 @@ def main():
     if args.ring_only:
         print(json.dumps({"pole_ring": run_ring(args)}), flush=True)
         return
-    if args.only in ("serve", "ensemble", "solve"):  # one leg alone (profiling, tuning)
+    if args.only in ("serve", "ensemble", "solve", "sizes"):  # one leg alone
         sys.path.insert(0, PKG)
-        if args.only == "solve":
+        if args.only == "sizes":
+            out = {"ensemble_sizes": run_ensemble_sizes(args)}
+        elif args.only == "solve":
             out = {"solve_path": run_solve_path(args)}
         elif args.only == "serve":
             import serve
