@@ -2817,6 +2817,7 @@ struct gk_solver {
   int ready;
   size_t bytes;
   int timing;
+  std::string time_only;  // time only this stage's launches (empty: all)
   int chunk;           // faces per thread in the marching sweeps
   int emf_chunk;       // edges per thread in the marching EMF kernels
   int emf_stage;       // bit D: E_D (D = 0, 1) over whole 64-edge k tiles staged in LDS
@@ -3544,6 +3545,7 @@ struct Timer {
   int slot;
   Timer(gk_solver *s_, const char *name) : s(s_), slot(-1) {
     if (!s->timing) return;
+    if (!s->time_only.empty() && s->time_only != name) return;
     for (size_t n = 0; n < s->names.size(); n++)
       if (s->names[n] == name) slot = (int)n;
     if (slot < 0) {
@@ -4340,6 +4342,14 @@ extern "C" int gk_set_time(gk_solver *s, double t) {
 extern "C" int gk_enable_timing(gk_solver *s, int on) {
   if (!s) return GK_ERR_ARG;
   s->timing = on;
+  s->time_only.clear();
+  return GK_OK;
+}
+
+extern "C" int gk_time_stage(gk_solver *s, const char *name) {
+  if (!s) return GK_ERR_ARG;
+  s->timing = 1;
+  s->time_only = name ? name : "";
   return GK_OK;
 }
 

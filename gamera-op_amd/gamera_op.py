@@ -65,6 +65,7 @@ def _load():
     _op.gk_get_scalars.restype = ctypes.c_int
     _op.gk_get_scalars.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
     _op.gk_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    _op.gk_time_stage.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     _op.gk_kernel_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p),
                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     _op.gk_debug_fetch.restype = ctypes.c_int
@@ -128,7 +129,7 @@ def exported_symbols():
     """Names of the C-ABI entry points declared in include/gamera_op.h."""
     return ["gk_create", "gk_destroy", "gk_set_state", "gk_prologue", "gk_step",
             "gk_get_state", "gk_get_scalars", "gk_set_time", "gk_enable_timing",
-            "gk_kernel_times", "gk_device_bytes"]
+            "gk_kernel_times", "gk_time_stage", "gk_device_bytes"]
 
 
 def run_main(config_file, nt):
@@ -294,6 +295,10 @@ class Problem:
 
     def enable_timing(self, on=True):
         self.op.gk_enable_timing(self.dev, 1 if on else 0)
+
+    def time_stage(self, name):
+        """Time only the launches of one stage (gk_time_stage)."""
+        _check(self.op.gk_time_stage(self.dev, name.encode() if name else None), "gk_time_stage")
 
     def kernel_times(self):
         n = ctypes.c_int(32)

@@ -189,8 +189,13 @@ int gk_get_scalars(gk_solver *s, double out[6]);
 int gk_set_time(gk_solver *s, double time_sim);
 
 /* Average kernel durations (ms) over the last gk_step call when timing is
-   enabled: names/ms arrays of length *n (in/out). */
+   enabled: names/ms arrays of length *n (in/out).  gk_time_stage turns timing
+   on for the launches of one stage only (name as gk_kernel_times reports it,
+   e.g. "k_cell"; NULL: every stage), so that a timed run carries two events
+   per step instead of two per launch; gk_enable_timing(s, 1) times every
+   stage again.  (No counterpart in the reference.) */
 int gk_enable_timing(gk_solver *s, int on);
+int gk_time_stage(gk_solver *s, const char *name);
 int gk_kernel_times(gk_solver *s, const char **names, double *ms, int *n);
 
 /* Copy one intermediate field of the last step to the host (reference
