@@ -494,6 +494,7 @@ def main(argv=None):
         out["served"]["tuning_s_before_stream"] = round(tuning, 3)
         out["served"]["loop_seconds"] = {k: round(v, 3) if isinstance(v, float) else v
                                          for k, v in serve.last_stats.items()}
+        out["served"]["gpu_busy_share"] = serve.last_stats.get("gpu_busy_s", 0.0) / wall
     if args.plain or args.no_serve:
         done[0] = 0
         res, wall = serve_plain(reqs, log=progress)
