@@ -282,6 +282,15 @@ __device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double qsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double qpow(double x, double y) { return pow(x, y); }
+// num == 0 and qdiv(G * num, den) is a zero (so that its square root squared
+// is +0): den not NaN and not 0 in FP64; in FP32, whose quotient multiplies
+// by a reciprocal, den also not subnormal (its reciprocal would overflow)
+__device__ __forceinline__ bool zero_ratio(double num, double den) {
+  return num == 0.0 && den == den && den != 0.0;
+}
+__device__ __forceinline__ bool zero_ratio(float num, float den) {
+  return num == 0.0f && fabsf(den) >= 1.17549435e-38f;
+}
 __device__ __forceinline__ float qpow(float x, float y) { return powf(x, y); }
 // a * b + c: in FP64 a rounded product, then a rounded sum (the reference's
 // operations; c + a * b gives the same bits); in FP32-bulk one fused
@@ -399,7 +408,15 @@ __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R b
   const R rl = L[0], rr = Rs[0], pl = L[4], pr = Rs[4], psl = L[5], psr = Rs[5];
   const R vl[3] = {L[1], L[2], L[3]}, vr[3] = {Rs[1], Rs[2], Rs[3]};
   R csl = qsqrt(qdiv(G * pl, rl)), csr = qsqrt(qdiv(G * pr, rr));
-  R cel = qsqrt(qdiv(G * psl, rl)), cer = qsqrt(qdiv(G * psr, rr));
+  // entropy sound speeds, used only squared: with p_S = 0 on both sides (every
+  // problem that does not set it) each square is +0 exactly, and a wave whose
+  // lanes all have it skips the divisions and square roots
+  R cel2 = R(0), cer2 = R(0);
+  if (!(zero_ratio(psl, rl) && zero_ratio(psr, rr))) {
+    const R cel = qsqrt(qdiv(G * psl, rl)), cer = qsqrt(qdiv(G * psr, rr));
+    cel2 = cel * cel;
+    cer2 = cer * cer;
+  }
   // sums of products are written with madd in the reference's order:
   // a + b * c as madd(b, c, a), a - b * c as madd(-b, c, a)
   const R tl[3] = {bl[0] + b0[0], bl[1] + b0[1], bl[2] + b0[2]};
@@ -412,8 +429,7 @@ __device__ __forceinline__ void face_flux(const R L[6], const R Rs[6], const R b
   R vnl = vl[D], vnr = vr[D];
   R VF = fmax(fabs(vnl) + qsqrt(madd(Vel, Vel, csl * csl)),
               fabs(vnr) + qsqrt(madd(Ver, Ver, csr * csr)));
-  R VFe = fmax(fabs(vnl) + qsqrt(madd(Vel, Vel, cel * cel)),
-               fabs(vnr) + qsqrt(madd(Ver, Ver, cer * cer)));
+  R VFe = fmax(fabs(vnl) + qsqrt(madd(Vel, Vel, cel2)), fabs(vnr) + qsqrt(madd(Ver, Ver, cer2)));
   R rvl[3], rvr[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) {
