@@ -297,6 +297,9 @@ __device__ __forceinline__ float qpow(float x, float y) { return powf(x, y); }
 // multiply-add (one rounding)
 __device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
 __device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// store of a value no later part of the same kernel reads (streaming)
+template <typename T>
+__device__ __forceinline__ void st_nt(T *p, T v) { __builtin_nontemporal_store(v, p); }
 // half-step value x + c (x - x_prev) of the AB2 predictor
 template <typename R>
 __device__ __forceinline__ R half_step(R x, R xp, R c) { return madd(c, x - xp, x); }
@@ -783,14 +786,14 @@ __device__ __forceinline__ void k_kcol_body(const SweepArgs &a, const int bx, co
   }
   if (!valid) return;
   R *F = (R *)a.F, *V = (R *)a.V;
-  V[c] = R(0.5) * (L[3] + Rs[3]);
-  V[fs + c] = R(0.5) * (L[1] + Rs[1]);
+  st_nt(V + c, R(0.5) * (L[3] + Rs[3]));
+  st_nt(V + fs + c, R(0.5) * (L[1] + Rs[1]));
   if (!full) return;
   R Fo[9], b0[4];
   face_b0<2, R, B0V>(a.b0, a.b0a, fs, c, b0);
   face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], b0, g.CA, Fo);
 #pragma unroll
-  for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
+  for (int m = 0; m < 9; m++) st_nt(F + m * fs + c, Fo[m]);
 }
 
 template <typename R, bool B0V>
@@ -954,13 +957,13 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
       for (int m = 0; m < 6; m++) march_lr(w, m, Tf, L[m], Rs[m]);
 #pragma unroll
       for (int m = 0; m < 3; m++) march_lr(w, 6 + m, Tf, bl[m], br[m]);
-      V[c] = R(0.5) * (L[va] + Rs[va]);
-      V[fs + c] = R(0.5) * (L[vb] + Rs[vb]);
+      st_nt(V + c, R(0.5) * (L[va] + Rs[va]));
+      st_nt(V + fs + c, R(0.5) * (L[vb] + Rs[vb]));
       R Fo[9], b0[4];
       face_b0<D, R, B0V>(a.b0, a.b0a, fs, c, b0);
       face_flux<D, R>(L, Rs, bl, br, bn, b0, g.CA, Fo);
 #pragma unroll
-      for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
+      for (int m = 0; m < 9; m++) st_nt(F + m * fs + c, Fo[m]);
       march_shift(w);
       c += sd;
       T += tstep;
@@ -1114,13 +1117,13 @@ __device__ __forceinline__ void k_sweepk_body(const SweepArgs &a, const int bx, 
   for (int m = 0; m < 6; m++) lr(m, L[m], Rs[m]);
 #pragma unroll
   for (int m = 0; m < 3; m++) lr(6 + m, bl[m], br[m]);
-  V[c] = R(0.5) * (L[3] + Rs[3]);
-  V[fs + c] = R(0.5) * (L[1] + Rs[1]);
+  st_nt(V + c, R(0.5) * (L[3] + Rs[3]));
+  st_nt(V + fs + c, R(0.5) * (L[1] + Rs[1]));
   R Fo[9], b0[4];
   face_b0<2, R, B0V>(a.b0, a.b0a, fs, c, b0);
   face_flux<2, R>(L, Rs, bl, br, ((const R *)a.B)[2 * fs + c], b0, g.CA, Fo);
 #pragma unroll
-  for (int m = 0; m < 9; m++) F[m * fs + c] = Fo[m];
+  for (int m = 0; m < 9; m++) st_nt(F + m * fs + c, Fo[m]);
 }
 
 template <typename R, bool B0V>
@@ -1484,7 +1487,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
       rsum = u0 + u1 + w0 + w1;
     double e0[3];
     edge_b0<D, B0V>(a.b0, a.b0a, fs, c, e0);
-    E[c] = emf_formula<D, R>(va, vb, b1l, b1r, b2l, b2r, rsum, e0, g.CA);
+    st_nt(E + c, emf_formula<D, R>(va, vb, b1l, b1r, b2l, b2r, rsum, e0, g.CA));
     u0 = w0;
     u1 = w1;
     if (VW) {
@@ -1600,20 +1603,20 @@ __device__ __forceinline__ void k_ct_body(const CtArgs &a, const int bx, const i
   if (aj && ak) {  // B_i on faces i = is..ie+1
     const double d = (ek_j * lk_j - ek * lk) - (ej_k * lj_k - ej * lj);
     const double b = bi - dt * d / si;
-    a.Bn[c] = b;
-    if (a.Bn32) a.Bn32[c] = (float)b;
+    st_nt(a.Bn + c, b);
+    if (a.Bn32) st_nt(a.Bn32 + c, (float)b);
   }
   if (ai && ak) {  // B_j on faces j = js..je+1
     const double d = (ei_k * li_k - ei * li) - (ek_i * lk_i - ek * lk);
     const double b = bj - dt * d / sj;
-    a.Bn[g.fs + c] = b;
-    if (a.Bn32) a.Bn32[g.fs + c] = (float)b;
+    st_nt(a.Bn + g.fs + c, b);
+    if (a.Bn32) st_nt(a.Bn32 + g.fs + c, (float)b);
   }
   if (ai && aj) {  // B_k on faces k = ks..ke+1
     const double d = (ej_i * lj_i - ej * lj) - (ei_j * li_j - ei * li);
     const double b = bk - dt * d / sk;
-    a.Bn[2 * g.fs + c] = b;
-    if (a.Bn32) a.Bn32[2 * g.fs + c] = (float)b;
+    st_nt(a.Bn + 2 * g.fs + c, b);
+    if (a.Bn32) st_nt(a.Bn32 + 2 * g.fs + c, (float)b);
   }
 }
 
@@ -1865,18 +1868,18 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
     double out[6] = {rhof, vn[0], vn[1], vn[2], p, ps};
 #pragma unroll
     for (int m = 0; m < 6; m++) {
-      a.Xn[m * g.fs + c] = out[m];
+      st_nt(a.Xn + m * g.fs + c, out[m]);
       bad |= !isfinite(out[m]);
     }
     if (a.Xn32) {
 #pragma unroll
-      for (int m = 0; m < 6; m++) a.Xn32[m * g.fs + c] = (float)out[m];
+      for (int m = 0; m < 6; m++) st_nt(a.Xn32 + m * g.fs + c, (float)out[m]);
     }
     // cell-centred B at n+1 (the ghost cells' is formed after the ghost fill)
 #pragma unroll
     for (int m = 0; m < 3; m++) {
-      if (a.Mn) a.Mn[m * g.fs + c] = bnew[m];
-      if (a.Mn32) a.Mn32[m * g.fs + c] = (float)bnew[m];
+      if (a.Mn) st_nt(a.Mn + m * g.fs + c, bnew[m]);
+      if (a.Mn32) st_nt(a.Mn32 + m * g.fs + c, (float)bnew[m]);
     }
     // next step's CFL dt from the new state (magtot = cell B + B0)
     double dcell = cfl_dt(rhof, vn[0], vn[1], vn[2], p, bnew[0] + cb0[0], bnew[1] + cb0[1],
