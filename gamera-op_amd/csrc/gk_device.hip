@@ -300,6 +300,9 @@ __device__ __forceinline__ float madd(float a, float b, float c) { return __buil
 // store of a value no later part of the same kernel reads (streaming)
 template <typename T>
 __device__ __forceinline__ void st_nt(T *p, T v) { __builtin_nontemporal_store(v, p); }
+// load of a value no other thread of the kernel reads (streaming)
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T *p) { return __builtin_nontemporal_load(p); }
 // half-step value x + c (x - x_prev) of the AB2 predictor
 template <typename R>
 __device__ __forceinline__ R half_step(R x, R xp, R c) { return madd(c, x - xp, x); }
@@ -1763,10 +1766,10 @@ __device__ __forceinline__ void k_cell_body(const CellArgs &a, const int bx, con
 
     double x[6];
 #pragma unroll
-    for (int m = 0; m < 6; m++) x[m] = a.X[m * g.fs + c];
+    for (int m = 0; m < 6; m++) x[m] = ld_nt(a.X + m * g.fs + c);
     double bold[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) bold[m] = (double)((const R *)a.M)[m * g.fs + c];
+    for (int m = 0; m < 3; m++) bold[m] = (double)ld_nt((const R *)a.M + m * g.fs + c);
     // face B at n+1 on the cell's six faces (cell-centred B and div B below),
     // loaded with the other inputs so the wave waits for memory once
     double bf[3][2];
