@@ -408,6 +408,10 @@ class Solver:
     def enable_timing(self, on=True):
         _check(self.op.gk_enable_timing(self.handle, 1 if on else 0), "gk_enable_timing")
 
+    def time_stage(self, name):
+        """Time only the launches of one stage (gk_time_stage; None: all)."""
+        _check(self.op.gk_time_stage(self.handle, name.encode() if name else None), "gk_time_stage")
+
     def kernel_times(self):
         """Average ms per launch of each stage kernel over the last step() call
         (with timing enabled)."""

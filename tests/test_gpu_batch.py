@@ -122,3 +122,45 @@ def test_autotune_keeps_results():
     finally:
         A.close()
         Bs.close()
+
+
+@pytest.mark.gpu
+def test_timing_controls():
+    """Per-stage timing: every stage, then one stage only (the others read
+    zero); batch run timing accumulates the device time of synced runs."""
+    P = go.Problem(go.SINUSOID_BLAST, 31, 32, 24, 16)
+    S = go.Solver(P, 0)
+    T = go.Solver(P, 1)
+    P.close()
+    B = None
+    try:
+        S.enable_timing(True)
+        S.step(3)
+        kt = S.kernel_times()
+        print("all stages", kt, flush=True)
+        assert kt["k_cell"] > 0.0 and kt["k_sweep_i"] > 0.0 and kt["k_ghost"] > 0.0
+        S.time_stage("k_cell")
+        S.step(3)
+        kt = S.kernel_times()
+        assert kt["k_cell"] > 0.0
+        assert all(v == 0.0 for k, v in kt.items() if k != "k_cell"), kt
+        S.enable_timing(False)
+        S.step(1)
+        B = go.Batch()
+        B.add(S)
+        B.add(T)
+        B.run_timing(True)
+        assert B.run_ms() == 0.0
+        B.step(6)
+        ms = B.run_ms()
+        assert ms > 0.0
+        B.step_async(2)
+        B.sync()
+        assert B.run_ms() > ms
+        B.run_timing(True)  # resets the total
+        assert B.run_ms() == 0.0
+    finally:
+        if B is not None:
+            B.close()
+        S.close()
+        T.close()
