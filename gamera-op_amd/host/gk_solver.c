@@ -16,6 +16,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <unistd.h>
 
 #include "common.h"
 #include "config.h"
@@ -778,6 +780,24 @@ int gkh_setup(int kind, uint64_t seed, int ni, int nj, int nk, int coord, int fu
   return 0;
 }
 
+/* Zero n bytes at p: the partial pages at either end are cleared, and the
+   whole pages between are handed back to the kernel (MADV_DONTNEED on the
+   private anonymous memory the field arrays live in), which maps zero pages
+   in again when they are next touched.  Clearing them one by one instead
+   made every page of every kept field fault in (0.84 s for a 128^3 grid's
+   fields, where a fresh set-up pays only for the fields it writes). */
+static void zero_fill(void *p, size_t n) {
+  const uintptr_t pg = (uintptr_t)sysconf(_SC_PAGESIZE);
+  const uintptr_t a = (uintptr_t)p, e = a + n;
+  const uintptr_t h = (a + pg - 1) / pg * pg, t = e / pg * pg;
+  if (t <= h || madvise((void *)h, t - h, MADV_DONTNEED) != 0) {
+    memset(p, 0, n);
+    return;
+  }
+  memset(p, 0, h - a);
+  memset((void *)t, 0, e - t);
+}
+
 /* host array access for the bindings: 0 gas f, 1 gem f, 2 geo f, 3 rec f,
    4.. coordinates x1 x2 x3 x1c x2c x3c x1ctr x2ctr x3ctr dx1 dx2 dx3 */
 /* New initial state for the problem already set up, with another seed: the
@@ -795,9 +815,9 @@ int gkh_reseed(uint64_t seed) {
   set_problem_config();
   const size_t n3 = (size_t)config.NI * config.NJ * config.NK;
   for (int f = 0; f < NF_gas; f++)
-    if (gas_keep[f] && gas[0][f]) memset(&gas[0][f][0][0][0], 0, n3 * sizeof(double));
+    if (gas_keep[f] && gas[0][f]) zero_fill(&gas[0][f][0][0][0], n3 * sizeof(double));
   for (int f = 0; f < NF_gem; f++)
-    if (gem_keep[f] && gem[f]) memset(&gem[f][0][0][0], 0, n3 * sizeof(double));
+    if (gem_keep[f] && gem[f]) zero_fill(&gem[f][0][0][0], n3 * sizeof(double));
   problem_init();
   set_background_field();
   if (gk_b0.mode < 0) return -3;
