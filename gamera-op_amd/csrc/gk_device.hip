@@ -3003,6 +3003,32 @@ static hipError_t upload(gk_solver *s, void *dst, const void *src, size_t bytes)
   return e;
 }
 
+// Default march lengths by grid size (faces per thread in the i and j
+// sweeps, edges per thread in the edge-field kernels): smaller grids need
+// shorter marches, so that their columns are split over enough threads to
+// fill the GPU.  From a sweep of 2..32 faces x 2..16 edges on one MI355X
+// (step times against the best pair): 32^3 (2, 2) best; 128x128x8 and
+// 64^3 (4, 2) within 3%; 96^3, 128^3 and 192x96x192 (16, 4) in FP64 and
+// (8, 4) in FP32-bulk within 3%; 256^3 (32, 16) best.  The pair that was the
+// default for every size, (32, 16), had cost 32^3 3.8x and 128x128x8 2.3x.
+// gk_autotune times the candidates on the grid itself.
+static void default_shapes(gk_solver *s) {
+  const double n = (double)s->ni * s->nj * s->nk;
+  if (n >= 4e6) {
+    s->chunk = 32;
+    s->emf_chunk = 16;
+  } else if (n >= 5e5) {
+    s->chunk = s->precision == 1 ? 8 : 16;
+    s->emf_chunk = 4;
+  } else if (n >= 1e5) {
+    s->chunk = 4;
+    s->emf_chunk = 2;
+  } else {
+    s->chunk = 2;
+    s->emf_chunk = 2;
+  }
+}
+
 extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   if (!d || !out) return GK_ERR_ARG;
   *out = nullptr;
@@ -3256,8 +3282,7 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
       hipSuccess)
     return fail(GK_ERR_DEVICE);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(GK_ERR_DEVICE);
-  s->chunk = 32;
-  s->emf_chunk = 16;
+  default_shapes(s);
   s->emf_stage = s->precision == 1 ? 2 : 3;  // measured fastest at 256^3
   s->cur = 0;
   s->b_m1 = 0;
