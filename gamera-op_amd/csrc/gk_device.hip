@@ -4600,9 +4600,22 @@ static int batch_build(gk_batch *b) {
   std::vector<char> hargs[NGROUP][6][SK_COUNT];
   std::vector<WorkItem> hwork[NGROUP][SK_COUNT];
   std::vector<Launch> plan;
+  // A batch that fills the GPU by itself (4e6 cells or more) needs no short
+  // marches to do so: its members march at least 16 faces and 4 edges per
+  // thread inside it (their own, autotuned lengths apply when stepped
+  // alone).  64-grid FP32-bulk ensemble: 6.39 ms per step with the members'
+  // own picks, 6.25 with this floor.
+  double cells = 0.0;
+  for (gk_solver *s : b->mem) cells += (double)s->ni * s->nj * s->nk;
+  const bool full = cells >= 4e6;
   for (gk_solver *s : b->mem) {
     const int gp = launch_group(s);
     Rot r = rot_of(s);
+    const int own_chunk = s->chunk, own_emf = s->emf_chunk;
+    if (full) {
+      s->chunk = std::max(s->chunk, 16);
+      s->emf_chunk = std::max(s->emf_chunk, 4);
+    }
     for (int p = 0; p < 6; p++) {
       plan_step(s, r, plan);
       for (const Launch &L : plan) {
@@ -4615,6 +4628,8 @@ static int batch_build(gk_batch *b) {
       }
       r = rot_next(r);
     }
+    s->chunk = own_chunk;
+    s->emf_chunk = own_emf;
   }
   size_t abytes = 0, nw = 0;
   for (int gp = 0; gp < NGROUP; gp++)
