@@ -74,6 +74,13 @@ struct Geo {
   const float *recwf[3];    // FP32 copies of recw (FP32-bulk form)
   const int *cls[3][2];
   int ncls[3][2];
+  // element strides of the reconstruction tables of axis a: tfs[a] from one
+  // face to the next, tps[a] from one class pair (class_b * ncls[a][1] +
+  // class_c) to the next.  Axes i and j are face-major (a wave's lanes, which
+  // run along k, share the face and differ in the k class); axis k is
+  // class-major (the lanes differ in the face and share the classes), so a
+  // wave reads contiguous rows either way.
+  long tfs[3], tps[3];
 };
 
 struct B0c {
@@ -314,7 +321,7 @@ __device__ __forceinline__ const R *rtab(const Geo &g, int f, const int idx[3]) 
   constexpr int B = A == 0 ? 1 : 0;
   constexpr int C = A == 2 ? 1 : 2;
   const int cb = g.cls[A][0][idx[B]], ccl = g.cls[A][1][idx[C]];
-  const long off = (((long)f * g.ncls[A][0] + cb) * g.ncls[A][1] + ccl) * GK_RECW_STRIDE;
+  const long off = (long)f * g.tfs[A] + ((long)cb * g.ncls[A][1] + ccl) * g.tps[A];
   if constexpr (sizeof(R) == 8)
     return (const R *)(g.recw[A] + off);
   else
@@ -892,7 +899,7 @@ __device__ __forceinline__ void k_march_body(const SweepArgs &a, const int bx, c
   long c = idx[0] * g.st[0] + idx[1] * g.st[1] + k;
   const R cc = (R)a.sc->c;
   const R *T = rtab<D, R>(a.geo, f0, idx);
-  const long tstep = (long)a.geo.ncls[D][0] * a.geo.ncls[D][1] * GK_RECW_STRIDE;
+  const long tstep = a.geo.tfs[D];
   constexpr int va = D == 0 ? 1 : 2;
   constexpr int vb = D == 0 ? 2 : 3;
   const long fs = g.fs;
@@ -1265,14 +1272,14 @@ struct TableWalk {
       t = (const R *)g.recw[A];
     else
       t = (const R *)g.recwf[A];
-    const long n0 = g.ncls[A][0], n1 = g.ncls[A][1];
+    const long n1 = g.ncls[A][1];
     if (MA == B) {  // fixed class on C
-      base = t + ((long)face * n0 * n1 + g.cls[A][1][idx[C]]) * GK_RECW_STRIDE;
-      mul = n1 * GK_RECW_STRIDE;
+      base = t + (long)face * g.tfs[A] + g.cls[A][1][idx[C]] * g.tps[A];
+      mul = n1 * g.tps[A];
       cls = g.cls[A][0];
     } else {        // fixed class on B
-      base = t + ((long)face * n0 + g.cls[A][0][idx[B]]) * n1 * GK_RECW_STRIDE;
-      mul = GK_RECW_STRIDE;
+      base = t + (long)face * g.tfs[A] + g.cls[A][0][idx[B]] * n1 * g.tps[A];
+      mul = g.tps[A];
       cls = g.cls[A][1];
     }
     next = cls[f0];
@@ -1349,7 +1356,7 @@ __device__ __forceinline__ void k_emf_march_body(const EmfArgs &a, const int bx,
     return half_step(x, (R)Xp[o], cc);
   };
   const R *Tm = rtab<MA, R>(a.geo, f0, idx);  // windowed reconstructions
-  const long tstep = (long)a.geo.ncls[MA][0] * a.geo.ncls[MA][1] * GK_RECW_STRIDE;
+  const long tstep = a.geo.tfs[MA];
   Window1<R> wv0, wv1;
   Window<1, R> wb;
   const R *Bw = VW ? B1 : B2, *Bwp = VW ? B1p : B2p;
@@ -3151,8 +3158,21 @@ extern "C" int gk_create(const gk_grid_desc *d, gk_solver **out) {
   bool cls_ok = true;
   for (int a = 0; a < 3; a++) {
     off_w[a] = h.size();
-    const size_t nw = (size_t)s->g.N[a] * d->ncls[a][0] * d->ncls[a][1] * GK_RECW_STRIDE;
-    h.insert(h.end(), d->recw[a], d->recw[a] + nw);
+    const size_t npair = (size_t)d->ncls[a][0] * d->ncls[a][1];
+    const size_t nf = s->g.N[a];
+    if (a < 2) {  // face-major, as given
+      h.insert(h.end(), d->recw[a], d->recw[a] + nf * npair * GK_RECW_STRIDE);
+      s->geo.tfs[a] = (long)npair * GK_RECW_STRIDE;
+      s->geo.tps[a] = GK_RECW_STRIDE;
+    } else {      // class-major
+      for (size_t p = 0; p < npair; p++)
+        for (size_t f = 0; f < nf; f++) {
+          const double *row = d->recw[a] + (f * npair + p) * GK_RECW_STRIDE;
+          h.insert(h.end(), row, row + GK_RECW_STRIDE);
+        }
+      s->geo.tfs[a] = GK_RECW_STRIDE;
+      s->geo.tps[a] = (long)nf * GK_RECW_STRIDE;
+    }
     const int tb = a == 0 ? 1 : 0, tc = a == 2 ? 1 : 2;
     const int tax[2] = {tb, tc};
     for (int q = 0; q < 2; q++) {
