@@ -3021,7 +3021,10 @@ static hipError_t upload(gk_solver *s, void *dst, const void *src, size_t bytes)
 // gk_autotune times the candidates on the grid itself.
 static void default_shapes(gk_solver *s) {
   const double n = (double)s->ni * s->nj * s->nk;
-  if (n >= 4e6) {
+  if (n >= 1.2e7) {
+    s->chunk = 64;
+    s->emf_chunk = 16;
+  } else if (n >= 4e6) {
     s->chunk = 32;
     s->emf_chunk = 16;
   } else if (n >= 5e5) {
@@ -3661,12 +3664,12 @@ void resolve_timers(gk_solver *s) {
 // Ids follow the order a step runs them in: a batch launches its stages in
 // id order (the k_pole_* and k_ring_* stages exist on pole ring-averaged grids)
 enum StageKernel {
-  SK_BEGIN = 0, SK_SWEEP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
+  SK_BEGIN = 0, SK_SWEEP_I, SK_VSTRIP_I, SK_SWEEP_J, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_I, SK_EMF_J,
   SK_EMF_K, SK_EMF_IS, SK_EMF_JS, SK_POLE_E, SK_CT, SK_POLE_B, SK_CELL, SK_RING_ROWS,
   SK_RING_CURL, SK_RING_FIX, SK_RING_MAGS, SK_GHOST, SK_MAGS, SK_RING_DT, SK_COUNT
 };
 static const char *const sk_name[SK_COUNT] = {
-    "k_begin", "k_sweep_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
+    "k_begin", "k_sweep_i", "k_vstrip_i", "k_sweep_j", "k_vstrip_j", "k_sweep_k", "k_krem_k", "k_emf_i",
     "k_emf_j", "k_emf_k", "k_emf_i_tiles", "k_emf_j_tiles", "k_pole_e", "k_ct", "k_pole_b",
     "k_cell", "k_ring_rows", "k_ring_curl", "k_ring_fix", "k_ring_mags", "k_ghost", "k_mags",
     "k_ring_dt"};
@@ -3686,7 +3689,8 @@ union StageArgs {
 static size_t stage_arg_size(int kid) {
   switch (kid) {
     case SK_BEGIN: return sizeof(Scal *);
-    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
+    case SK_SWEEP_I: case SK_VSTRIP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K:
+    case SK_KREM:
       return sizeof(SweepArgs);
     case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: case SK_EMF_IS: case SK_EMF_JS:
       return sizeof(EmfArgs);
@@ -3704,7 +3708,8 @@ static size_t stage_arg_size(int kid) {
 static const void *stage_arg_ptr(int kid, const StageArgs &u) {
   switch (kid) {
     case SK_BEGIN: return &u.sc;
-    case SK_SWEEP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K: case SK_KREM:
+    case SK_SWEEP_I: case SK_VSTRIP_I: case SK_SWEEP_J: case SK_VSTRIP_J: case SK_SWEEP_K:
+    case SK_KREM:
       return &u.sw;
     case SK_EMF_I: case SK_EMF_J: case SK_EMF_K: case SK_EMF_IS: case SK_EMF_JS:
       return &u.em;
@@ -3794,24 +3799,53 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     const int X2 = D == 0 ? 1 : (D == 1 ? 2 : 0);
     a.lo[X2] = 0;
     a.hi[X2] = g.N[X2] - 2;
+    // on grids whose i and j marches still fill every wave slot (1024 SIMDs
+    // x 64 lanes x the march's waves per SIMD: one in FP64, two in FP32)
+    // with twice the march length, they walk twice the length: there the
+    // window prologue costs more than the longer march loses in parallelism
+    // (256^3 at the default of 64: 128 faces per thread, 2 and 3 chunks of
+    // 1024 waves); smaller grids stepped alone keep the parallelism
+    auto long_march = [&](SweepArgs &m, int cols) {
+      const int n2 = march_chunks(m.hi[D] - m.lo[D] + 1, std::min(2 * ch, 4096));
+      if ((long)cols * n2 < 1024L * 64 * (f32 ? 2 : 1)) return false;
+      m.nch = n2;
+      return true;
+    };
     if (D == 0) {
+      // on such grids the i march covers the active rows only (j inside the
+      // box), so that the launch is a whole number of waves per chunk
+      // (256^3: 1024 waves of 64 k points), and the j ghost rows, which only
+      // need face velocities, go to velocity-only strips, as the k ghost
+      // columns of the j sweep do; on smaller grids (and batch members) the
+      // two extra launches would cost more than they save
+      SweepArgs m = a;
+      m.lo[1] = S[1];
+      m.hi[1] = Ee[1];
+      if (!long_march(m, (m.hi[2] - m.lo[2] + 1) * (m.hi[1] - m.lo[1] + 1))) {
+        L.kid = SK_SWEEP_I;
+        L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[1], a.hi[1]), 1, a.nch);
+        L.u.sw = a;
+        out.push_back(L);
+        continue;
+      }
       L.kid = SK_SWEEP_I;
-      L.grid = dim3(row_blocks(a.lo[2], a.hi[2], a.lo[1], a.hi[1]), 1, a.nch);
-      L.u.sw = a;
+      L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[1], m.hi[1]), 1, m.nch);
+      L.u.sw = m;
       out.push_back(L);
+      for (int side = 0; side < 2; side++) {
+        SweepArgs v = a;
+        v.lo[1] = side == 0 ? 0 : Ee[1] + 1;
+        v.hi[1] = side == 0 ? S[1] - 1 : g.N[1] - 2;
+        L.kid = SK_VSTRIP_I;  // NGH = 4 rows on both sides
+        L.grid = dim3(point_blocks(v), 1, 1);
+        L.u.sw = v;
+        out.push_back(L);
+      }
     } else if (D == 1) {
       SweepArgs m = a;  // full faces: k over the active range
       m.lo[2] = S[2];
       m.hi[2] = Ee[2];
-      // on grids that still fill every wave slot of the march (1024 SIMDs
-      // x 64 lanes x its waves per SIMD: one in FP64, two in FP32), the j
-      // march walks twice the i march's length: there its prologue costs
-      // more than the longer march loses in parallelism (256^3: 64 faces per
-      // thread fastest for j, 32 for i, in both forms); smaller grids
-      // stepped alone keep the parallelism
-      const long cols = (long)(m.hi[2] - m.lo[2] + 1) * (m.hi[0] - m.lo[0] + 1);
-      const int n2 = march_chunks(m.hi[1] - m.lo[1] + 1, std::min(2 * ch, 4096));
-      if (cols * n2 >= 1024L * 64 * (f32 ? 2 : 1)) m.nch = n2;
+      long_march(m, (m.hi[2] - m.lo[2] + 1) * (m.hi[0] - m.lo[0] + 1));
       L.kid = SK_SWEEP_J;
       L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1, m.nch);
       L.u.sw = m;
@@ -4073,6 +4107,7 @@ static void launch_one(const Launch &L, hipStream_t st) {
   switch (L.kid) {
     case SK_BEGIN: k_begin<<<1, NSLOT, 0, st>>>(L.u.sc); break;
     case SK_SWEEP_I: k_march<0, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
+    case SK_VSTRIP_I: k_sweep<0, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_J: k_march<1, R, B0V, UT><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_VSTRIP_J: k_sweep<1, R, B0V><<<g1, blk, 0, st>>>(L.u.sw, gx, gy, gz); break;
     case SK_SWEEP_K: k_sweepk<R, B0V><<<g1, dim3(64), 0, st>>>(L.u.sw, gx, gy, gz); break;
@@ -4103,6 +4138,7 @@ static void launch_batch(int kid, const void *args, const WorkItem *work, int nw
   switch (kid) {
     case SK_BEGIN: k_begin_batch<<<nwork, NSLOT, 0, st>>>((Scal *const *)args, nwork); break;
     case SK_SWEEP_I: k_march_batch<0, R, B0V, UT><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
+    case SK_VSTRIP_I: k_sweep_batch<0, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_J: k_march_batch<1, R, B0V, UT><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_VSTRIP_J: k_sweep_batch<1, R, B0V><<<nblocks, blk, 0, st>>>((const SweepArgs *)args, work, nwork); break;
     case SK_SWEEP_K: k_sweepk_batch<R, B0V><<<nblocks, dim3(64), 0, st>>>((const SweepArgs *)args, work, nwork); break;
@@ -4582,7 +4618,7 @@ static void solo_order(int kid, dim3 gr, int idx, std::vector<WorkItem> &out) {
     case SK_CT: case SK_CELL:
       mode = gy == 1 ? 2 : 0;
       break;
-    default:  // SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_K, SK_MAGS, the ring stages
+    default:  // SK_VSTRIP_I, SK_VSTRIP_J, SK_SWEEP_K, SK_KREM, SK_EMF_K, SK_MAGS, the ring stages
       mode = 0;
   }
   if (mode == 2) {
