@@ -3022,8 +3022,12 @@ static hipError_t upload(gk_solver *s, void *dst, const void *src, size_t bytes)
 static void default_shapes(gk_solver *s) {
   const double n = (double)s->ni * s->nj * s->nk;
   if (n >= 1.2e7) {
+    // 17 edges per thread: 257 edge positions in 15 chunks, whose launches
+    // end nearer a whole number of rounds of the wave slots than 16 chunks
+    // (256^3 FP64: E_i and E_j tiles 16384 -> 15360 waves, 5 rounds of
+    // 3072 slots; 0.3% faster steps in both forms)
     s->chunk = 64;
-    s->emf_chunk = 16;
+    s->emf_chunk = 17;
   } else if (n >= 4e6) {
     s->chunk = 32;
     s->emf_chunk = 16;

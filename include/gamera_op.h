@@ -217,7 +217,7 @@ size_t gk_device_bytes(const gk_solver *s);
    many edges each thread walks in the edge-field kernels (1..4096).  Their
    defaults depend on the grid's cell count: 2 and 2 below 1e5 cells, 4 and
    2 below 5e5, 16 (FP32-bulk: 8) and 4 below 4e6, 32 and 16 below 1.2e7,
-   64 and 16 above.  A column is split into near-equal chunks of
+   64 and 17 above.  A column is split into near-equal chunks of
    about that length (within 1.5x of it).  GK_TUNE_EMF_STAGE: bit D (D = 0
    for E_i, 1 for E_j) runs that edge field over whole 64-edge tiles along
    k with its k-direction reconstructions staged in LDS (0..3; default 3 in
