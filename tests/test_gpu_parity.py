@@ -101,3 +101,25 @@ def test_launch_shapes_match_oracle(kind, seed, ni, nj, nk, steps):
             assert P.device_scalars()["time"] == O.scalar("time_sim")
     finally:
         P.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("chunk", [1, 64])
+def test_split_i_sweep_matches_oracle(chunk):
+    """At 64^3 FP64 one face per thread is short enough that the i and j
+    marches fill every wave slot at twice the length: the i march then covers
+    the active j rows only and velocity strips the j ghost rows (the layout a
+    256^3 grid takes at its default lengths); 64 faces per thread keeps the
+    single i march over all rows.  Both bitwise equal to the oracle."""
+    print(f"split i sweep 64^3 chunk={chunk}", flush=True)
+    P = go.Problem(go.SINUSOID_SPHERE, 17, 64, 64, 64, full_b0=True)
+    try:
+        O = run_oracle(P, 3)
+        P.device_open(0)
+        P.set_tuning(go.TUNE_CHUNK, chunk)
+        P.device_step(3)
+        P.device_fetch()
+        assert exact_mismatches(P, O) == 0
+        assert P.device_scalars()["time"] == O.scalar("time_sim")
+    finally:
+        P.close()
