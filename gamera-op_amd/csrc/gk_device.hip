@@ -3839,8 +3839,8 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
     // x 64 lanes x the march's waves per SIMD: one in FP64, two in FP32)
     // with twice the march length, they walk twice the length: there the
     // window prologue costs more than the longer march loses in parallelism
-    // (256^3 at the default of 64: 128 faces per thread, 2 and 3 chunks of
-    // 1024 waves); smaller grids stepped alone keep the parallelism
+    // (256^3 at the default of 64: 128-129 faces per thread, 2 chunks of
+    // 1024 waves each); smaller grids stepped alone keep the parallelism
     auto long_march = [&](SweepArgs &m, int cols) {
       const int n2 = march_chunks(m.hi[D] - m.lo[D] + 1, std::min(2 * ch, 4096));
       if ((long)cols * n2 < 1024L * 64 * (f32 ? 2 : 1)) return false;
