@@ -651,6 +651,8 @@ struct SweepArgs {
   int lo[3], hi[3];
   int flo[3], fhi[3];    // range where the full flux is needed
   int nch;               // marching sweeps: near-equal chunks of the march range
+  int gap;               // velocity strips: strip-axis indices from NGH on are
+                         // shifted by gap (both ghost strips in one box)
 };
 
 __device__ __forceinline__ void globalize(SweepArgs &a) {
@@ -679,10 +681,12 @@ __device__ __forceinline__ void k_sweep_body(const SweepArgs &a, const int bx, c
   const int W = a.hi[2] - a.lo[2] + 1, H = a.hi[1] - a.lo[1] + 1;
   const int p = bx * 256 + threadIdx.y * 64 + threadIdx.x;
   const int t = p / W;
-  const int k = a.lo[2] + (p - t * W);
-  const int j = a.lo[1] + t % H;
+  int k = a.lo[2] + (p - t * W);
+  int j = a.lo[1] + t % H;
   const int i = a.lo[0] + t / H;
   if (i > a.hi[0]) return;
+  if (D == 0 && j >= NGH) j += a.gap;
+  if (D == 1 && k >= NGH) k += a.gap;
   const Grid &g = a.g;
   const long c = i * g.st[0] + j * g.st[1] + k;
   const long sd = g.st[D];
@@ -3918,11 +3922,14 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[1], m.hi[1]), 1, m.nch);
       L.u.sw = m;
       out.push_back(L);
-      for (int side = 0; side < 2; side++) {
+      {
+        // both strips (NGH rows below the box and NGH above) in one box of
+        // 2 NGH virtual rows, the upper ones shifted past the box
         SweepArgs v = a;
-        v.lo[1] = side == 0 ? 0 : Ee[1] + 1;
-        v.hi[1] = side == 0 ? S[1] - 1 : g.N[1] - 2;
-        L.kid = SK_VSTRIP_I;  // NGH = 4 rows on both sides
+        v.lo[1] = 0;
+        v.hi[1] = 2 * NGH - 1;
+        v.gap = Ee[1] + 1 - NGH;
+        L.kid = SK_VSTRIP_I;
         L.grid = dim3(point_blocks(v), 1, 1);
         L.u.sw = v;
         out.push_back(L);
@@ -3936,12 +3943,14 @@ static void plan_step(const gk_solver *s, const Rot &r, std::vector<Launch> &out
       L.grid = dim3(row_blocks(m.lo[2], m.hi[2], m.lo[0], m.hi[0]), 1, m.nch);
       L.u.sw = m;
       out.push_back(L);
-      // velocity-only strips k in [0, ks-1] and [ke+1, NK-2]
-      for (int side = 0; side < 2; side++) {
+      // velocity-only strips k in [0, ks-1] and [ke+1, NK-2], as one box
+      // of 2 NGH virtual columns, the upper ones shifted past the box
+      {
         SweepArgs v = a;
-        v.lo[2] = side == 0 ? 0 : Ee[2] + 1;
-        v.hi[2] = side == 0 ? S[2] - 1 : g.N[2] - 2;
-        L.kid = SK_VSTRIP_J;  // strip width is NGH = 4 on both sides
+        v.lo[2] = 0;
+        v.hi[2] = 2 * NGH - 1;
+        v.gap = Ee[2] + 1 - NGH;
+        L.kid = SK_VSTRIP_J;
         L.grid = dim3(point_blocks(v), 1, 1);
         L.u.sw = v;
         out.push_back(L);
